@@ -1,0 +1,274 @@
+"""Benchmark of the data-parallel primitives on one MI355X.
+
+Default line (the driver's contract): one step = one multi-pass gather plus
+one multi-pass scatter of 2^28 int32 values through a full random
+permutation (12 algorithmic bytes per element each: index, source and
+destination once), inputs resident in HBM, pass count from the library's
+schedule.  ``value`` is GB/s (1e9 B/s) of algorithmic bytes.
+
+``--suite`` adds every primitive at its full size (STREAM, scans, gather /
+scatter over pass counts and index kinds, stable split over bucket counts and
+key skews) under the key "suite" of the same JSON line.
+
+Timing: HIP events recorded on the library context's stream, which is torch's
+current stream, around exactly K steps after W warm-up steps; per-kernel
+averages come from events around the individual calls.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "GB/s & % HBM peak at 2^28 int32; batched+learned vs plain per-call, bit-exact"
+HBM_PEAK_GBS = 8000.0
+N_FULL = 1 << 28
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--suite", action="store_true", help="measure every primitive too")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--cpu-sample", type=int, default=1 << 26, help="elements in the CPU baseline sample")
+    p.add_argument("--passes", type=int, default=0, help="pass count of the default step (0 = library schedule)")
+    p.add_argument("--out", default=None, help="also write the JSON line to this file")
+    return p.parse_args()
+
+
+def hampel(x):
+    """average_Hampel (opencl/util/utility.cpp:38-83) restated for the report."""
+    x = np.asarray(x, np.float64)
+    med = np.median(x)
+    d = np.abs(x - med)
+    mad = np.median(d) / 0.6745
+    keep = (d == 0) if mad == 0 else (d / mad <= 3.5)
+    return float(x[keep].mean())
+
+
+class Timer:
+    """HIP events on torch's current stream (the library context's stream)."""
+
+    def __init__(self, torch):
+        self.torch = torch
+
+    def time(self, fn, reps, warm=2):
+        torch = self.torch
+        for _ in range(warm):
+            fn()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        torch.cuda.synchronize()
+        for a, b in evs:
+            a.record()
+            fn()
+            b.record()
+        torch.cuda.synchronize()
+        ms = [a.elapsed_time(b) for a, b in evs]
+        return {"median_ms": float(np.median(ms)), "hampel_ms": hampel(ms), "min_ms": float(np.min(ms))}
+
+
+def gbs(nbytes, ms):
+    return nbytes / (ms * 1e-3) / 1e9
+
+
+def cpu_baseline(args, torch_unused=None):
+    """The reference's OpenMP gather/scatter (oracle/_ref) or the port, on host cores."""
+    from oracle import pyoracle as orc
+    n = args.cpu_sample
+    idx = orc.gen_perm(n, 0xC0FFEE, 0)
+    src = orc.gen_i32(n, 0xBEEF)
+    out = np.empty(n, np.int32)
+    ref = orc.reference()
+    b = orc.baseline()
+    threads = b.baseline_threads()
+    if ref is not None:
+        kind = "reference"
+        g = lambda: ref.edpp_ref_gather(orc._a(src), orc._a(out), orc._a(idx), n)
+        s = lambda: ref.edpp_ref_scatter(orc._a(src), orc._a(out), orc._a(idx), n)
+    else:
+        kind = "port"
+        g = lambda: b.baseline_gather(orc._a(src), orc._a(out), orc._a(idx), n)
+        s = lambda: b.baseline_scatter(orc._a(src), orc._a(out), orc._a(idx), n)
+    g(), s()  # first touch, discarded (the reference discards run 0 in its bandwidth tests)
+    times = []
+    t_end = time.time() + 20.0
+    while len(times) < 5 or (time.time() < t_end and len(times) < 20):
+        times.append(g() + s())
+        if len(times) >= 5 and time.time() > t_end:
+            break
+    ms = hampel(times)
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {
+        "value": round(gbs(24.0 * n, ms), 3),
+        "unit": "GB/s",
+        "cores": int(threads),
+        "kind": kind,
+        "sample": f"one OpenMP gather + one scatter of 2^{int(np.log2(n))} int32 through a random permutation, "
+                  f"Hampel mean of {len(times)} runs ({ms:.1f} ms); host {cpu}, nproc={os.cpu_count()}, "
+                  f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')}",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is design-only.",
+              file=sys.stderr)
+        sys.exit(2)
+    import importlib
+    import torch
+    edpp = importlib.import_module("efficient-dpp_amd")
+    torch.cuda.set_device(0)
+    ctx = edpp.Context(0, stream=torch.cuda.current_stream().cuda_stream)
+    T = Timer(torch)
+    n = N_FULL
+    i32 = torch.int32
+
+    # ---- inputs of the step, resident in HBM before timing
+    idx = torch.empty(n, dtype=i32, device="cuda")
+    ctx.gen_perm(idx, 0x5EED, edpp.PERM_RANDOM)
+    src = torch.empty(n, dtype=i32, device="cuda")
+    ctx.gen_i32(src, 0xDA7A)
+    dst = torch.empty(n, dtype=i32, device="cuda")
+    dst2 = torch.empty(n, dtype=i32, device="cuda")
+    passes = args.passes
+
+    def step():
+        ctx.gather(src, dst, idx, n, passes)
+        ctx.scatter(dst, dst2, idx, n, passes)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    total_ms = e0.elapsed_time(e1)
+    ms_step = total_ms / args.steps
+    value = gbs(24.0 * n, ms_step)
+
+    # bit-exact spot check of the step's result (scatter undoes gather)
+    assert torch.equal(dst2, src), "gather/scatter round trip mismatch"
+
+    # dominant kernel: the gather launch(es); per-launch average from events
+    g = T.time(lambda: ctx.gather(src, dst, idx, n, passes), reps=max(5, args.steps))
+    sc = T.time(lambda: ctx.scatter(dst, dst2, idx, n, passes), reps=max(5, args.steps))
+    eff_passes = passes if passes > 0 else 1
+    achieved = gbs(12.0 * n, g["hampel_ms"])
+    copy = T.time(lambda: ctx.stream_op(edpp.STREAM_COPY, src, None, dst, n), reps=10)
+    copy_gbs = gbs(8.0 * n, copy["hampel_ms"])
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (SplitMix64 values, keyed-Feistel random permutation; generated on device)",
+        "config": {"workload": "multi-pass gather + scatter, 2^28 int32, full random permutation",
+                   "n": n, "passes": eff_passes if passes else "schedule", "index": "random permutation",
+                   "bytes_per_element_per_primitive": 12},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_gather (one launch per pass)",
+                     "avg_launch_ms": round(g["hampel_ms"] / eff_passes, 4),
+                     "vs_stream_copy": round(achieved / copy_gbs, 4), "stream_copy_gbs": round(copy_gbs, 2)},
+        "per_primitive": {"gather_gbs": round(achieved, 2), "scatter_gbs": round(gbs(12.0 * n, sc["hampel_ms"]), 2)},
+    }
+    del dst2
+    if args.suite:
+        line["suite"] = suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs)
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args)
+    s = json.dumps(line)
+    print(s, flush=True)
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(s + "\n")
+    ctx.close()
+
+
+def suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs):
+    n = N_FULL
+    res = {}
+    i32 = torch.int32
+    # STREAM (opencl/test/test_bandwidth.cpp: 2^28 int32 per array)
+    b = torch.empty(n, dtype=i32, device="cuda")
+    ctx.gen_i32(b, 0x11)
+    st = {}
+    for name, op, nb in [("copy", 0, 8), ("scale", 1, 8), ("add", 2, 12), ("triad", 3, 12)]:
+        t = T.time(lambda: ctx.stream_op(op, src, b, dst, n), reps=10)
+        st[name] = round(gbs(nb * n, t["hampel_ms"]), 2)
+    res["stream_gbs"] = st
+    print("stream", st, file=sys.stderr, flush=True)
+    # scans: C1 (2^24, values in [0,15]) and the metric size 2^28
+    sc = {}
+    for lg in (24, 28):
+        m = 1 << lg
+        x = b[:m]
+        ctx.gen_i32(x, 2024, 15)
+        for name, scheme in [("lookback", 1), ("rts", 2), ("ssa", 3)]:
+            t = T.time(lambda: ctx.scan(x, dst, m, scheme), reps=10)
+            g = gbs(8.0 * m, t["hampel_ms"])
+            sc[f"2^{lg}_{name}"] = {"gbs": round(g, 2), "ms": round(t["hampel_ms"], 4),
+                                    "frac_peak": round(g / HBM_PEAK_GBS, 4), "frac_copy": round(g / copy_gbs, 4)}
+    res["scan"] = sc
+    print("scan", sc, file=sys.stderr, flush=True)
+    # gather / scatter over pass counts and index kinds
+    gs = {}
+    for kind, kname in [(edpp.PERM_RANDOM, "random"), (edpp.PERM_BLOCK_LOCAL, "block4096")]:
+        ctx.gen_perm(idx, 0x5EED, kind)
+        for p in (1, 2, 4, 8, 16):
+            tg = T.time(lambda: ctx.gather(src, dst, idx, n, p), reps=5)
+            ts = T.time(lambda: ctx.scatter(src, dst, idx, n, p), reps=5)
+            gs[f"{kname}_p{p}"] = {"gather_gbs": round(gbs(12.0 * n, tg["hampel_ms"]), 2),
+                                   "scatter_gbs": round(gbs(12.0 * n, ts["hampel_ms"]), 2)}
+    ctx.gen_perm(idx, 0x5EED, edpp.PERM_RANDOM)
+    res["gather_scatter"] = gs
+    print("gather_scatter", gs, file=sys.stderr, flush=True)
+    # stable split, 2^28 (key, value) pairs
+    sp = {}
+    ko = torch.empty(n, dtype=i32, device="cuda")
+    vo = torch.empty(n, dtype=i32, device="cuda")
+    start = torch.empty(4096, dtype=i32, device="cuda")
+    for B in (2, 16, 256, 4096):
+        for dist, dname in [(0, "uniform"), (1, "zipf")]:
+            ctx.gen_split_keys(src, b, B, dist, 0x5917 + B)
+            t = T.time(lambda: ctx.split(src, ko, B, b, vo, start, n, edpp.SPLIT_WG), reps=5)
+            g = gbs(16.0 * n, t["hampel_ms"])
+            sp[f"B{B}_{dname}"] = {"gbs": round(g, 2), "ms": round(t["hampel_ms"], 3),
+                                   "frac_copy": round(g / copy_gbs, 4)}
+            print("split", B, dname, sp[f"B{B}_{dname}"], file=sys.stderr, flush=True)
+    res["split_kv"] = sp
+    del ko, vo, b
+    ctx.gen_i32(src, 0xDA7A)
+    return res
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,199 @@
+"""Python host layer over the C ABI of libedpp.so (include/edpp.h).
+
+The package directory name contains a hyphen, so import it with
+``importlib.import_module("efficient-dpp_amd")``.  Device buffers are any
+objects exposing ``data_ptr()`` (torch tensors on the GPU) or plain integer
+addresses; torch is used only for allocation and streams.  There is no CPU
+fallback: when libedpp.so is missing every call raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libedpp.so")
+
+# ---- status codes and enums (mirror include/edpp.h) -----------------------
+OK, EINVAL, ENOMEM, ETIMEOUT, EHIP, ENODEV, EUNSUP = 0, -1, -2, -3, -4, -5, -6
+
+SCAN_AUTO, SCAN_LOOKBACK, SCAN_RTS, SCAN_SSA = 0, 1, 2, 3
+MATRIX_LM, MATRIX_REG, MATRIX_LM_REG, MATRIX_LM_SERIAL = 0, 1, 2, 3
+NET_SERIAL, NET_BLELLOCH, NET_SKLANSKY, NET_WAVE = 0, 1, 2, 3
+SPLIT_AUTO, SPLIT_WG, SPLIT_WI, SPLIT_SINGLE = 0, 1, 2, 3
+STREAM_COPY, STREAM_SCALE, STREAM_ADD, STREAM_TRIAD = 0, 1, 2, 3
+PERM_RANDOM, PERM_BLOCK_LOCAL, PERM_IDENTITY = 0, 1, 2
+KEYS_UNIFORM, KEYS_ZIPF = 0, 1
+
+_lib = None
+
+c_i32, c_i64, c_u64, c_int, c_vp, c_dp = (ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_int,
+                                          ctypes.c_void_p, ctypes.POINTER(ctypes.c_double))
+
+# name -> argtypes (all return int unless listed in _RESTYPES)
+_SIGS = {
+    "edpp_abi_version": [],
+    "edpp_strerror": [c_int],
+    "edpp_last_error": [c_vp],
+    "edpp_ctx_create": [c_int, c_vp, ctypes.POINTER(c_vp)],
+    "edpp_ctx_destroy": [c_vp],
+    "edpp_ctx_sync": [c_vp],
+    "edpp_ctx_stream": [c_vp],
+    "edpp_ctx_device_info": [c_vp, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_i64),
+                             ctypes.POINTER(c_i64)],
+    "edpp_stream_i32": [c_vp, c_int, c_vp, c_vp, c_vp, c_i64, c_dp],
+    "edpp_gather_i32": [c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_dp],
+    "edpp_scatter_i32": [c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_dp],
+    "edpp_gather_i64": [c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_dp],
+    "edpp_scatter_i64": [c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_dp],
+    "edpp_scan_exclusive_i32": [c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_dp],
+    "edpp_scan_exclusive_i64": [c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_dp],
+    "edpp_scan_local_i32": [c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_dp],
+    "edpp_split_i32": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_dp],
+    "edpp_split_i64": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_dp],
+    "edpp_gen_i32": [c_vp, c_vp, c_i64, c_u64, c_i32],
+    "edpp_gen_i64": [c_vp, c_vp, c_i64, c_u64, c_i64],
+    "edpp_gen_perm": [c_vp, c_vp, c_i64, c_u64, c_int],
+    "edpp_gen_split_keys_i32": [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_u64],
+    "edpp_gen_split_keys_i64": [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_u64],
+}
+_RESTYPES = {"edpp_strerror": ctypes.c_char_p, "edpp_last_error": ctypes.c_char_p, "edpp_ctx_stream": c_vp}
+
+
+def declared_symbols():
+    """Every entry point this binding declares (checked against the headers by the tests)."""
+    return sorted(_SIGS)
+
+
+def lib():
+    """Load libedpp.so once; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run efficient-dpp_amd/build.py (no CPU fallback exists)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RESTYPES.get(name, c_int)
+        _lib = L
+    return _lib
+
+
+class EdppError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"edpp status {status}: {msg}")
+        self.status = status
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def _numel(x):
+    return x.numel() if hasattr(x, "numel") else None
+
+
+class Context:
+    """One library context: a HIP stream, a workspace and look-back state.
+
+    ``stream`` is a raw hipStream_t address (for example
+    ``torch.cuda.current_stream().cuda_stream``) or None for a private stream.
+    """
+
+    def __init__(self, device=0, stream=None):
+        self._L = lib()
+        h = c_vp()
+        rc = self._L.edpp_ctx_create(int(device), stream, ctypes.byref(h))
+        if rc != OK:
+            raise EdppError(rc, self._L.edpp_strerror(rc).decode())
+        self.h = h
+        cus, xcds, l2, mall = c_int(), c_int(), c_i64(), c_i64()
+        self._L.edpp_ctx_device_info(self.h, ctypes.byref(cus), ctypes.byref(xcds), ctypes.byref(l2),
+                                     ctypes.byref(mall))
+        self.cus, self.xcds, self.l2_bytes, self.mall_bytes = cus.value, xcds.value, l2.value, mall.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.edpp_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- plumbing
+    def _check(self, rc):
+        if rc != OK:
+            raise EdppError(rc, self._L.edpp_last_error(self.h).decode())
+
+    def _call(self, name, *args, timed=False):
+        ms = ctypes.c_double(0.0) if timed else None
+        self._check(getattr(self._L, name)(self.h, *args, ctypes.byref(ms) if timed else None))
+        return ms.value if timed else None
+
+    def sync(self):
+        self._check(self._L.edpp_ctx_sync(self.h))
+
+    @property
+    def stream(self):
+        return self._L.edpp_ctx_stream(self.h)
+
+    # -- primitives (n defaults to the element count of the first buffer)
+    def stream_op(self, op, a, b, c, n=None, timed=False):
+        n = _numel(a) if n is None else n
+        return self._call("edpp_stream_i32", int(op), _ptr(a), _ptr(b), _ptr(c), n, timed=timed)
+
+    def gather(self, src, dst, idx, n=None, passes=0, timed=False, wide=False):
+        n = _numel(idx) if n is None else n
+        name = "edpp_gather_i64" if wide else "edpp_gather_i32"
+        return self._call(name, _ptr(src), _ptr(dst), _ptr(idx), n, int(passes), timed=timed)
+
+    def scatter(self, src, dst, idx, n=None, passes=0, timed=False, wide=False):
+        n = _numel(idx) if n is None else n
+        name = "edpp_scatter_i64" if wide else "edpp_scatter_i32"
+        return self._call(name, _ptr(src), _ptr(dst), _ptr(idx), n, int(passes), timed=timed)
+
+    def scan(self, src, dst, n=None, scheme=SCAN_AUTO, total=None, timed=False, wide=False):
+        n = _numel(src) if n is None else n
+        name = "edpp_scan_exclusive_i64" if wide else "edpp_scan_exclusive_i32"
+        return self._call(name, _ptr(src), _ptr(dst), n, int(scheme), _ptr(total), timed=timed)
+
+    def scan_local(self, src, dst, tile_size, local_size=512, matrix=MATRIX_LM_REG, network=NET_WAVE, timed=False):
+        return self._call("edpp_scan_local_i32", _ptr(src), _ptr(dst), int(tile_size), int(local_size), int(matrix),
+                          int(network), timed=timed)
+
+    def split(self, keys_in, keys_out, buckets, vals_in=None, vals_out=None, start=None, n=None,
+              strategy=SPLIT_AUTO, timed=False, wide=False):
+        n = _numel(keys_in) if n is None else n
+        name = "edpp_split_i64" if wide else "edpp_split_i32"
+        return self._call(name, _ptr(keys_in), _ptr(keys_out), _ptr(vals_in), _ptr(vals_out), _ptr(start), n,
+                          int(buckets), int(strategy), timed=timed)
+
+    # -- reproducible inputs
+    def gen_i32(self, out, seed, mask=-1, n=None):
+        self._check(self._L.edpp_gen_i32(self.h, _ptr(out), _numel(out) if n is None else n, seed & (2**64 - 1),
+                                         mask))
+
+    def gen_i64(self, out, seed, mask=-1, n=None):
+        self._check(self._L.edpp_gen_i64(self.h, _ptr(out), _numel(out) if n is None else n, seed & (2**64 - 1),
+                                         mask))
+
+    def gen_perm(self, idx, seed, kind=PERM_RANDOM, n=None):
+        self._check(self._L.edpp_gen_perm(self.h, _ptr(idx), _numel(idx) if n is None else n, seed & (2**64 - 1),
+                                          int(kind)))
+
+    def gen_split_keys(self, keys, vals, buckets, dist, seed, n=None, wide=False):
+        name = "edpp_gen_split_keys_i64" if wide else "edpp_gen_split_keys_i32"
+        self._check(getattr(self._L, name)(self.h, _ptr(keys), _ptr(vals), _numel(keys) if n is None else n,
+                                           int(buckets), int(dist), seed & (2**64 - 1)))

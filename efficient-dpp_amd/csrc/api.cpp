@@ -1,0 +1,388 @@
+// api.cpp — context management and the C ABI entry points of include/edpp.h.
+//
+// Argument checking follows the reference's conventions where it has them
+// (opencl/primitives/scanImpl.cpp:19-22 rejects R==L==0;
+// opencl/primitives/splitImpl.cpp:29-45 rejects missing SOA values) but every
+// failure is a status code: nothing here exits the process.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+#include <new>
+#include "ctx.hpp"
+
+namespace edpp {
+
+int fail(edpp_ctx* c, int code, const char* fmt, ...) {
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        c->last_error = buf;
+    }
+    return code;
+}
+
+int hip_fail(edpp_ctx* c, hipError_t e, const char* what) {
+    return fail(c, EDPP_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int ws_reserve(edpp_ctx* c, size_t bytes, void** out) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (bytes > c->ws_bytes) {
+        if (c->ws) {
+            EDPP_HIP(c, hipStreamSynchronize(c->stream));  // earlier kernels may still use it
+            EDPP_HIP(c, hipFree(c->ws));
+            c->ws = nullptr;
+            c->ws_bytes = 0;
+        }
+        size_t want = std::max(bytes, (size_t)1 << 20);
+        if (hipMalloc(&c->ws, want) != hipSuccess) {
+            c->ws = nullptr;
+            return fail(c, EDPP_ENOMEM, "workspace of %zu bytes", want);
+        }
+        c->ws_bytes = want;
+    }
+    *out = c->ws;
+    return EDPP_OK;
+}
+
+int scan_state_acquire(edpp_ctx* c, int64_t tiles, uint32_t* epoch_out) {
+    if (tiles > c->scan_cap_tiles) {
+        int64_t cap = std::max<int64_t>(tiles, 1 << 16);
+        if (c->scan_status) {
+            EDPP_HIP(c, hipStreamSynchronize(c->stream));
+            hipFree(c->scan_status);
+            hipFree(c->scan_flags);
+            hipFree(c->scan_vals);
+            c->scan_status = nullptr;
+            c->scan_flags = nullptr;
+            c->scan_vals = nullptr;
+        }
+        if (hipMalloc(&c->scan_status, cap * sizeof(uint64_t)) != hipSuccess ||
+            hipMalloc(&c->scan_flags, cap * sizeof(uint32_t)) != hipSuccess ||
+            hipMalloc(&c->scan_vals, cap * 2 * sizeof(uint64_t)) != hipSuccess) {
+            c->scan_cap_tiles = 0;
+            return fail(c, EDPP_ENOMEM, "look-back state for %lld tiles", (long long)cap);
+        }
+        EDPP_HIP(c, hipMemsetAsync(c->scan_status, 0, cap * sizeof(uint64_t), c->stream));
+        EDPP_HIP(c, hipMemsetAsync(c->scan_flags, 0, cap * sizeof(uint32_t), c->stream));
+        EDPP_HIP(c, hipMemsetAsync(c->tickets, 0, 2 * sizeof(uint32_t), c->stream));
+        c->scan_cap_tiles = cap;
+        c->epoch = 1;
+    }
+    if (c->epoch >= (1u << 30) - 1) {
+        // epochs are 30-bit: clear the state once per 2^30 calls
+        EDPP_HIP(c, hipMemsetAsync(c->scan_status, 0, c->scan_cap_tiles * sizeof(uint64_t), c->stream));
+        EDPP_HIP(c, hipMemsetAsync(c->scan_flags, 0, c->scan_cap_tiles * sizeof(uint32_t), c->stream));
+        EDPP_HIP(c, hipMemsetAsync(c->tickets, 0, 2 * sizeof(uint32_t), c->stream));
+        c->epoch = 1;
+    }
+    *epoch_out = c->epoch++;
+    return EDPP_OK;
+}
+
+int timing_begin(edpp_ctx* c, double* ms) {
+    if (ms) EDPP_HIP(c, hipEventRecord(c->ev_start, c->stream));
+    return EDPP_OK;
+}
+
+int check_device_error(edpp_ctx* c) {
+    int32_t err = 0;
+    EDPP_HIP(c, hipMemcpy(&err, c->d_err, sizeof err, hipMemcpyDeviceToHost));
+    if (err) {
+        int32_t zero = 0;
+        EDPP_HIP(c, hipMemcpy(c->d_err, &zero, sizeof zero, hipMemcpyHostToDevice));
+        return fail(c, err, "device watchdog reported status %d", err);
+    }
+    return EDPP_OK;
+}
+
+int timing_end(edpp_ctx* c, double* ms) {
+    if (!ms) return EDPP_OK;
+    EDPP_HIP(c, hipEventRecord(c->ev_stop, c->stream));
+    EDPP_HIP(c, hipEventSynchronize(c->ev_stop));
+    float f = 0.f;
+    EDPP_HIP(c, hipEventElapsedTime(&f, c->ev_start, c->ev_stop));
+    *ms = (double)f;
+    return check_device_error(c);
+}
+
+// ---------------------------------------------------------------- schedules
+// Defaults used when the caller passes 0 / AUTO and no tuned schedule exists.
+int default_passes(edpp_ctx* c, int64_t n, int elem_bytes, bool scatter) {
+    (void)c; (void)n; (void)elem_bytes; (void)scatter;
+    return 1;
+}
+int default_scan_scheme(edpp_ctx* c, int64_t n, int elem_bytes) {
+    (void)c; (void)n; (void)elem_bytes;
+    return EDPP_SCAN_LOOKBACK;
+}
+int default_split_strategy(edpp_ctx* c, int64_t n, int buckets) {
+    (void)c; (void)n; (void)buckets;
+    return EDPP_SPLIT_WG;
+}
+
+}  // namespace edpp
+
+using namespace edpp;
+
+#define EDPP_CHECK_CTX(c) \
+    do { if (!(c)) return EDPP_EINVAL; } while (0)
+
+extern "C" {
+
+int edpp_abi_version(void) { return EDPP_ABI_VERSION; }
+
+const char* edpp_strerror(int s) {
+    switch (s) {
+        case EDPP_OK: return "ok";
+        case EDPP_EINVAL: return "invalid argument";
+        case EDPP_ENOMEM: return "out of device memory";
+        case EDPP_ETIMEOUT: return "device wait timed out";
+        case EDPP_EHIP: return "HIP runtime error";
+        case EDPP_ENODEV: return "no HIP device";
+        case EDPP_EUNSUP: return "unsupported combination";
+        default: return "unknown status";
+    }
+}
+
+const char* edpp_last_error(edpp_ctx* c) { return c ? c->last_error.c_str() : "no context"; }
+
+int edpp_ctx_create(int device, void* stream, edpp_ctx** out) {
+    if (!out) return EDPP_EINVAL;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return EDPP_ENODEV;
+    if (device < 0 || device >= count) return EDPP_EINVAL;
+    edpp_ctx* c = new (std::nothrow) edpp_ctx();
+    if (!c) return EDPP_ENOMEM;
+    c->device = device;
+    int rc = EDPP_OK;
+    do {
+        if (hipSetDevice(device) != hipSuccess) { rc = EDPP_EHIP; break; }
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) != hipSuccess) { rc = EDPP_EHIP; break; }
+        c->cus = prop.multiProcessorCount;
+        c->l2_bytes = prop.l2CacheSize;
+        int xcc = 0;
+        if (hipDeviceGetAttribute(&xcc, hipDeviceAttributeNumberOfXccs, device) == hipSuccess && xcc > 0)
+            c->xcds = xcc;
+        // Infinity Cache of MI355X (not reported by the runtime)
+        c->mall_bytes = (int64_t)256 << 20;
+        if (stream) {
+            c->stream = (hipStream_t)stream;
+        } else {
+            if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { rc = EDPP_EHIP; break; }
+            c->own_stream = true;
+        }
+        if (hipEventCreate(&c->ev_start) != hipSuccess || hipEventCreate(&c->ev_stop) != hipSuccess) {
+            rc = EDPP_EHIP;
+            break;
+        }
+        if (hipMalloc(&c->tickets, 64) != hipSuccess || hipMalloc(&c->d_err, 64) != hipSuccess) {
+            rc = EDPP_ENOMEM;
+            break;
+        }
+        if (hipMemset(c->tickets, 0, 64) != hipSuccess || hipMemset(c->d_err, 0, 64) != hipSuccess) {
+            rc = EDPP_EHIP;
+            break;
+        }
+    } while (0);
+    if (rc) {
+        edpp_ctx_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return EDPP_OK;
+}
+
+int edpp_ctx_destroy(edpp_ctx* c) {
+    if (!c) return EDPP_OK;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    hipFree(c->ws);
+    hipFree(c->scan_status);
+    hipFree(c->scan_flags);
+    hipFree(c->scan_vals);
+    hipFree(c->tickets);
+    hipFree(c->d_err);
+    if (c->ev_start) hipEventDestroy(c->ev_start);
+    if (c->ev_stop) hipEventDestroy(c->ev_stop);
+    if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return EDPP_OK;
+}
+
+int edpp_ctx_sync(edpp_ctx* c) {
+    EDPP_CHECK_CTX(c);
+    EDPP_HIP(c, hipStreamSynchronize(c->stream));
+    return check_device_error(c);
+}
+
+void* edpp_ctx_stream(edpp_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int edpp_ctx_device_info(edpp_ctx* c, int* cus, int* xcds, int64_t* l2, int64_t* mall) {
+    EDPP_CHECK_CTX(c);
+    if (cus) *cus = c->cus;
+    if (xcds) *xcds = c->xcds;
+    if (l2) *l2 = c->l2_bytes;
+    if (mall) *mall = c->mall_bytes;
+    return EDPP_OK;
+}
+
+// ---------------------------------------------------------------- STREAM
+int edpp_stream_i32(edpp_ctx* c, int op, const int32_t* a, const int32_t* b, int32_t* out, int64_t n, double* ms) {
+    EDPP_CHECK_CTX(c);
+    if (n < 0 || (n > 0 && (!a || !out))) return fail(c, EDPP_EINVAL, "stream: null pointer or n < 0");
+    if ((op == EDPP_STREAM_ADD || op == EDPP_STREAM_TRIAD) && n > 0 && !b)
+        return fail(c, EDPP_EINVAL, "stream add/triad need b");
+    if (op < EDPP_STREAM_COPY || op > EDPP_STREAM_TRIAD) return fail(c, EDPP_EINVAL, "unknown stream op %d", op);
+    int rc = timing_begin(c, ms);
+    if (!rc) rc = launch_stream_i32(c, op, a, b, out, n);
+    if (!rc) rc = timing_end(c, ms);
+    return rc;
+}
+
+// ---------------------------------------------------------------- gather / scatter
+}  // extern "C"
+
+template <class T>
+static int gs_common(edpp_ctx* c, const T* in, T* out, const int32_t* idx, int64_t n, int passes, double* ms,
+                     bool scatter) {
+    EDPP_CHECK_CTX(c);
+    if (n < 0 || n >= (1ll << 31)) return fail(c, EDPP_EINVAL, "n=%lld outside [0, 2^31)", (long long)n);
+    if (n > 0 && (!in || !out || !idx)) return fail(c, EDPP_EINVAL, "null buffer");
+    if (passes < 0) return fail(c, EDPP_EINVAL, "pass count %d < 0", passes);
+    if (n > 0 && (const void*)in == (const void*)out) return fail(c, EDPP_EINVAL, "in and out must differ");
+    int rc = timing_begin(c, ms);
+    if (!rc) rc = launch_gather<T>(c, in, out, idx, n, passes, scatter);
+    if (!rc) rc = timing_end(c, ms);
+    return rc;
+}
+
+extern "C" {
+
+int edpp_gather_i32(edpp_ctx* c, const int32_t* in, int32_t* out, const int32_t* idx, int64_t n, int passes,
+                    double* ms) {
+    return gs_common<uint32_t>(c, (const uint32_t*)in, (uint32_t*)out, idx, n, passes, ms, false);
+}
+int edpp_scatter_i32(edpp_ctx* c, const int32_t* in, int32_t* out, const int32_t* idx, int64_t n, int passes,
+                     double* ms) {
+    return gs_common<uint32_t>(c, (const uint32_t*)in, (uint32_t*)out, idx, n, passes, ms, true);
+}
+int edpp_gather_i64(edpp_ctx* c, const int64_t* in, int64_t* out, const int32_t* idx, int64_t n, int passes,
+                    double* ms) {
+    return gs_common<uint64_t>(c, (const uint64_t*)in, (uint64_t*)out, idx, n, passes, ms, false);
+}
+int edpp_scatter_i64(edpp_ctx* c, const int64_t* in, int64_t* out, const int32_t* idx, int64_t n, int passes,
+                     double* ms) {
+    return gs_common<uint64_t>(c, (const uint64_t*)in, (uint64_t*)out, idx, n, passes, ms, true);
+}
+
+// ---------------------------------------------------------------- scans
+}  // extern "C"
+
+template <class T>
+static int scan_common(edpp_ctx* c, const T* in, T* out, int64_t n, int scheme, T* d_total, double* ms) {
+    EDPP_CHECK_CTX(c);
+    if (n < 0 || n >= (1ll << 40)) return fail(c, EDPP_EINVAL, "n=%lld outside [0, 2^40)", (long long)n);
+    if (n > 0 && (!in || !out)) return fail(c, EDPP_EINVAL, "null buffer");
+    if (scheme < EDPP_SCAN_AUTO || scheme > EDPP_SCAN_SSA) return fail(c, EDPP_EINVAL, "unknown scheme %d", scheme);
+    int rc = timing_begin(c, ms);
+    if (!rc) rc = launch_scan<T>(c, in, out, n, scheme, d_total);
+    if (!rc) rc = timing_end(c, ms);
+    return rc;
+}
+
+extern "C" {
+
+int edpp_scan_exclusive_i32(edpp_ctx* c, const int32_t* in, int32_t* out, int64_t n, int scheme, int32_t* d_total,
+                            double* ms) {
+    return scan_common<uint32_t>(c, (const uint32_t*)in, (uint32_t*)out, n, scheme, (uint32_t*)d_total, ms);
+}
+int edpp_scan_exclusive_i64(edpp_ctx* c, const int64_t* in, int64_t* out, int64_t n, int scheme, int64_t* d_total,
+                            double* ms) {
+    return scan_common<uint64_t>(c, (const uint64_t*)in, (uint64_t*)out, n, scheme, (uint64_t*)d_total, ms);
+}
+
+int edpp_scan_local_i32(edpp_ctx* c, const int32_t* in, int32_t* out, int tile_size, int local_size, int matrix_type,
+                        int network, double* ms) {
+    EDPP_CHECK_CTX(c);
+    if (!in || !out) return fail(c, EDPP_EINVAL, "null buffer");
+    int rc = timing_begin(c, ms);
+    if (!rc) rc = launch_scan_local(c, in, out, tile_size, local_size, matrix_type, network);
+    if (!rc) rc = timing_end(c, ms);
+    return rc;
+}
+
+// ---------------------------------------------------------------- split
+}  // extern "C"
+
+template <class K>
+static int split_common(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t* vout, int32_t* d_start,
+                        int64_t n, int buckets, int strategy, double* ms) {
+    EDPP_CHECK_CTX(c);
+    if (n < 0 || n >= (1ll << 31)) return fail(c, EDPP_EINVAL, "n=%lld outside [0, 2^31)", (long long)n);
+    if (buckets < 1 || buckets > 4096 || (buckets & (buckets - 1)))
+        return fail(c, EDPP_EINVAL, "buckets=%d must be a power of two in [1,4096]", buckets);
+    if (n > 0 && (!kin || !kout)) return fail(c, EDPP_EINVAL, "null key buffer");
+    if ((vin == nullptr) != (vout == nullptr))
+        return fail(c, EDPP_EINVAL, "values: both or neither of vals_in/vals_out");  // splitImpl.cpp:29-34
+    if (n > 0 && (const void*)kin == (const void*)kout) return fail(c, EDPP_EINVAL, "keys in and out must differ");
+    if (n > 0 && vin && (const void*)vin == (const void*)vout) return fail(c, EDPP_EINVAL, "vals in and out must differ");
+    if (strategy < EDPP_SPLIT_AUTO || strategy > EDPP_SPLIT_SINGLE)
+        return fail(c, EDPP_EINVAL, "unknown split strategy %d", strategy);
+    int rc = timing_begin(c, ms);
+    if (!rc) rc = launch_split<K>(c, kin, kout, vin, vout, d_start, n, buckets, strategy);
+    if (!rc) rc = timing_end(c, ms);
+    return rc;
+}
+
+extern "C" {
+
+int edpp_split_i32(edpp_ctx* c, const int32_t* kin, int32_t* kout, const int32_t* vin, int32_t* vout,
+                   int32_t* d_start, int64_t n, int buckets, int strategy, double* ms) {
+    return split_common<uint32_t>(c, (const uint32_t*)kin, (uint32_t*)kout, vin, vout, d_start, n, buckets, strategy,
+                                  ms);
+}
+int edpp_split_i64(edpp_ctx* c, const int64_t* kin, int64_t* kout, const int32_t* vin, int32_t* vout,
+                   int32_t* d_start, int64_t n, int buckets, int strategy, double* ms) {
+    return split_common<uint64_t>(c, (const uint64_t*)kin, (uint64_t*)kout, vin, vout, d_start, n, buckets, strategy,
+                                  ms);
+}
+
+// ---------------------------------------------------------------- generators
+int edpp_gen_i32(edpp_ctx* c, int32_t* out, int64_t n, uint64_t seed, int32_t mask) {
+    EDPP_CHECK_CTX(c);
+    if (n < 0 || (n > 0 && !out)) return fail(c, EDPP_EINVAL, "gen: bad buffer");
+    return launch_gen_u32(c, (uint32_t*)out, n, seed, (uint32_t)mask);
+}
+int edpp_gen_i64(edpp_ctx* c, int64_t* out, int64_t n, uint64_t seed, int64_t mask) {
+    EDPP_CHECK_CTX(c);
+    if (n < 0 || (n > 0 && !out)) return fail(c, EDPP_EINVAL, "gen: bad buffer");
+    return launch_gen_u64(c, (uint64_t*)out, n, seed, (uint64_t)mask);
+}
+int edpp_gen_perm(edpp_ctx* c, int32_t* idx, int64_t n, uint64_t seed, int kind) {
+    EDPP_CHECK_CTX(c);
+    if (n < 0 || n >= (1ll << 31) || (n > 0 && !idx)) return fail(c, EDPP_EINVAL, "gen_perm: bad buffer or n");
+    return launch_gen_perm(c, idx, n, seed, kind);
+}
+int edpp_gen_split_keys_i32(edpp_ctx* c, int32_t* keys, int32_t* vals, int64_t n, int buckets, int dist,
+                            uint64_t seed) {
+    EDPP_CHECK_CTX(c);
+    if (n < 0 || n >= (1ll << 31) || (n > 0 && !keys)) return fail(c, EDPP_EINVAL, "gen_split_keys: bad buffer");
+    if (buckets < 1 || buckets > 4096 || (buckets & (buckets - 1))) return fail(c, EDPP_EINVAL, "bad bucket count");
+    return launch_gen_split_keys<uint32_t>(c, (uint32_t*)keys, vals, n, buckets, dist, seed);
+}
+int edpp_gen_split_keys_i64(edpp_ctx* c, int64_t* keys, int32_t* vals, int64_t n, int buckets, int dist,
+                            uint64_t seed) {
+    EDPP_CHECK_CTX(c);
+    if (n < 0 || n >= (1ll << 31) || (n > 0 && !keys)) return fail(c, EDPP_EINVAL, "gen_split_keys: bad buffer");
+    if (buckets < 1 || buckets > 4096 || (buckets & (buckets - 1))) return fail(c, EDPP_EINVAL, "bad bucket count");
+    return launch_gen_split_keys<uint64_t>(c, (uint64_t*)keys, vals, n, buckets, dist, seed);
+}
+
+}  // extern "C"

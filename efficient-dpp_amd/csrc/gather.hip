@@ -1,0 +1,165 @@
+// gather.hip — multi-pass gather and scatter.
+//
+// Reference: opencl/primitives/gatherImpl.cpp:7-55, scatterImpl.cpp:9-54
+// (host pass loop), opencl/kernels/gather_kernel.cl:6-19,
+// scatter_kernel.cl:6-21 (kernels).  Semantics kept: pass k of P handles the
+// indices inside [k*ceil(n/P), (k+1)*ceil(n/P)), so the random-side working
+// set of one pass is n/P elements and can stay in L2 / Infinity Cache while
+// the sequential index stream is re-read every pass.
+//
+// MI355X layout: each lane owns U groups of 4 consecutive positions; the
+// index (and, for scatter, value) groups are read as 16-byte vectors with
+// non-temporal hints so the streaming side does not evict the random side.
+#include "ctx.hpp"
+#include "device.hpp"
+
+namespace edpp {
+
+constexpr int kGsBlock = 256;
+
+template <class T>
+__device__ __forceinline__ void load4(const T* p, T (&v)[4]) {
+    if (sizeof(T) == 4) {
+        v4u32 x = ld_stream(reinterpret_cast<const v4u32*>(p));
+        #pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = (T)x[k];
+    } else {
+        v2u64 a = ld_stream(reinterpret_cast<const v2u64*>(p));
+        v2u64 b = ld_stream(reinterpret_cast<const v2u64*>(p) + 1);
+        v[0] = (T)a[0]; v[1] = (T)a[1]; v[2] = (T)b[0]; v[3] = (T)b[1];
+    }
+}
+template <class T>
+__device__ __forceinline__ void store4(T* p, const T (&v)[4]) {
+    if (sizeof(T) == 4) {
+        v4u32 x;
+        #pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = (uint32_t)v[k];
+        st_stream(x, reinterpret_cast<v4u32*>(p));
+    } else {
+        v2u64 a, b;
+        a[0] = v[0]; a[1] = v[1]; b[0] = v[2]; b[1] = v[3];
+        st_stream(a, reinterpret_cast<v2u64*>(p));
+        st_stream(b, reinterpret_cast<v2u64*>(p) + 1);
+    }
+}
+
+// out[i] = in[idx[i]] for idx[i] in [from, from+span).  The host clamps
+// from+span to n, so an index outside [0,n) is skipped instead of read.
+template <class T, int U>
+__global__ __launch_bounds__(kGsBlock) void k_gather(const T* __restrict__ in, T* __restrict__ out,
+                                                     const int32_t* __restrict__ idx, int64_t ngrp, uint32_t from,
+                                                     uint32_t span) {
+    const int64_t g0 = (int64_t)blockIdx.x * kGsBlock * U + threadIdx.x;
+    uint32_t pos[U][4];
+    #pragma unroll
+    for (int u = 0; u < U; ++u) {
+        int64_t g = g0 + (int64_t)u * kGsBlock;
+        v4u32 x = {from + span, from + span, from + span, from + span};  // out of range
+        if (g < ngrp) x = ld_stream(reinterpret_cast<const v4u32*>(idx) + g);
+        #pragma unroll
+        for (int k = 0; k < 4; ++k) pos[u][k] = x[k];
+    }
+    T val[U][4];
+    #pragma unroll
+    for (int u = 0; u < U; ++u)
+        #pragma unroll
+        for (int k = 0; k < 4; ++k) val[u][k] = (pos[u][k] - from < span) ? in[pos[u][k]] : T(0);
+    #pragma unroll
+    for (int u = 0; u < U; ++u) {
+        int64_t g = g0 + (int64_t)u * kGsBlock;
+        if (g < ngrp) {
+            bool all = true;
+            #pragma unroll
+            for (int k = 0; k < 4; ++k) all = all && (pos[u][k] - from < span);
+            if (all) {
+                store4(out + 4 * g, val[u]);
+            } else {
+                #pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (pos[u][k] - from < span) out[4 * g + k] = val[u][k];
+            }
+        }
+    }
+}
+
+// out[idx[i]] = in[i] for idx[i] in [from, from+span)
+template <class T, int U>
+__global__ __launch_bounds__(kGsBlock) void k_scatter(const T* __restrict__ in, T* __restrict__ out,
+                                                      const int32_t* __restrict__ idx, int64_t ngrp, uint32_t from,
+                                                      uint32_t span) {
+    const int64_t g0 = (int64_t)blockIdx.x * kGsBlock * U + threadIdx.x;
+    uint32_t pos[U][4];
+    T val[U][4];
+    #pragma unroll
+    for (int u = 0; u < U; ++u) {
+        int64_t g = g0 + (int64_t)u * kGsBlock;
+        if (g < ngrp) {
+            v4u32 x = ld_stream(reinterpret_cast<const v4u32*>(idx) + g);
+            #pragma unroll
+            for (int k = 0; k < 4; ++k) pos[u][k] = x[k];
+            load4(in + 4 * g, val[u]);
+        } else {
+            #pragma unroll
+            for (int k = 0; k < 4; ++k) { pos[u][k] = from + span; val[u][k] = 0; }
+        }
+    }
+    #pragma unroll
+    for (int u = 0; u < U; ++u)
+        #pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (pos[u][k] - from < span) out[pos[u][k]] = val[u][k];
+}
+
+// elements [lo, n): scalar tail and the path for unaligned pointers
+template <class T>
+__global__ void k_gs_scalar(const T* in, T* out, const int32_t* idx, int64_t lo, int64_t n, uint32_t from,
+                            uint32_t span, bool scatter) {
+    for (int64_t i = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t p = (uint32_t)idx[i];
+        if (p - from < span) {
+            if (scatter) out[p] = in[i];
+            else out[i] = in[p];
+        }
+    }
+}
+
+template <class T>
+int launch_gather(edpp_ctx* c, const T* in, T* out, const int32_t* idx, int64_t n, int passes, bool scatter) {
+    if (n == 0) return EDPP_OK;
+    if (passes == 0) passes = default_passes(c, n, (int)sizeof(T), scatter);
+    if (passes < 1 || passes > n) passes = passes < 1 ? 1 : (int)n;
+    constexpr int U = 4;
+    const bool vec = aligned16(in) && aligned16(out) && aligned16(idx);
+    const int64_t ngrp = vec ? n / 4 : 0;
+    const int64_t len = ceil_div(n, passes);  // gatherImpl.cpp:37
+    const unsigned grid = (unsigned)ceil_div(ngrp, (int64_t)kGsBlock * U);
+    for (int p = 0; p < passes; ++p) {
+        const int64_t lo = (int64_t)p * len;
+        if (lo >= n) break;
+        const uint32_t from = (uint32_t)lo;
+        const uint32_t span = (uint32_t)(lo + len <= n ? len : n - lo);
+        if (ngrp) {
+            if (scatter)
+                hipLaunchKernelGGL((k_scatter<T, U>), dim3(grid), dim3(kGsBlock), 0, c->stream, in, out, idx, ngrp,
+                                   from, span);
+            else
+                hipLaunchKernelGGL((k_gather<T, U>), dim3(grid), dim3(kGsBlock), 0, c->stream, in, out, idx, ngrp,
+                                   from, span);
+        }
+        if (ngrp * 4 < n) {
+            int64_t rest = n - ngrp * 4;
+            unsigned g = (unsigned)(rest < 256 * 4096 ? ceil_div(rest, 256) : 4096);
+            hipLaunchKernelGGL((k_gs_scalar<T>), dim3(g), dim3(256), 0, c->stream, in, out, idx, ngrp * 4, n, from,
+                               span, scatter);
+        }
+    }
+    EDPP_HIP(c, hipGetLastError());
+    return EDPP_OK;
+}
+
+template int launch_gather<uint32_t>(edpp_ctx*, const uint32_t*, uint32_t*, const int32_t*, int64_t, int, bool);
+template int launch_gather<uint64_t>(edpp_ctx*, const uint64_t*, uint64_t*, const int32_t*, int64_t, int, bool);
+
+}  // namespace edpp

@@ -1,0 +1,190 @@
+// scan_local.hip — single-workgroup tile scans ("matrix scans") and the
+// intra-workgroup networks that scan the per-lane partials.
+//
+// Reference: opencl/kernels/scan_local_kernel.cl:478-695 (scan_local and
+// matrix_scan_lm / _reg / _lm_reg / _lm_serial), networks :42-259 (serial,
+// Blelloch, Sklansky, warp scan), driver opencl/test/test_scan_local.cpp:6-106
+// (one workgroup of 512 lanes scanning 512*TILE ints, exclusive).  Each lane
+// owns a row of `tile` consecutive elements: reduce the row, scan the row
+// sums across the workgroup with the chosen network, rescan the row.  The
+// warp network of the reference assumes 32-lane lockstep without barriers;
+// the wave network here uses DPP on 64-lane waves and one LDS exchange.
+#include "ctx.hpp"
+#include "device.hpp"
+
+namespace edpp {
+
+constexpr int kMaxRow = 64;    // largest tile (elements per lane)
+constexpr int kMaxLocal = 1024;
+
+// exclusive scan of part[0..L) in LDS, in place; returns the total (all lanes)
+__device__ uint32_t scan_partials_network(uint32_t* part, int L, int network, uint32_t* wtmp) {
+    const int t = threadIdx.x;
+    uint32_t total = 0;
+    if (network == EDPP_NET_SERIAL) {
+        if (t == 0) {
+            uint32_t acc = 0;
+            for (int i = 0; i < L; ++i) {
+                uint32_t x = part[i];
+                part[i] = acc;
+                acc += x;
+            }
+            wtmp[0] = acc;
+        }
+        __syncthreads();
+        total = wtmp[0];
+    } else if (network == EDPP_NET_BLELLOCH) {
+        // up-sweep
+        int stride = 1;
+        for (; stride < L; stride <<= 1) {
+            int i = (t + 1) * stride * 2 - 1;
+            if (i < L) part[i] += part[i - stride];
+            __syncthreads();
+        }
+        if (t == 0) {
+            wtmp[0] = part[L - 1];
+            part[L - 1] = 0;
+        }
+        __syncthreads();
+        // down-sweep
+        for (stride = L >> 1; stride >= 1; stride >>= 1) {
+            int i = (t + 1) * stride * 2 - 1;
+            if (i < L) {
+                uint32_t x = part[i - stride];
+                part[i - stride] = part[i];
+                part[i] += x;
+            }
+            __syncthreads();
+        }
+        total = wtmp[0];
+    } else if (network == EDPP_NET_SKLANSKY) {
+        uint32_t own = t < L ? part[t] : 0u;
+        for (int d = 1; d < L; d <<= 1) {
+            uint32_t add = 0;
+            if (t < L && (t & d)) add = part[(t & ~(d - 1)) - 1];  // last element of the left block
+            __syncthreads();
+            if (t < L && (t & d)) part[t] += add;
+            __syncthreads();
+        }
+        total = part[L - 1];
+        __syncthreads();
+        if (t < L) part[t] -= own;
+        __syncthreads();
+    } else {  // EDPP_NET_WAVE
+        const int w = t / kWave, lane = lane_id();
+        uint32_t x = t < L ? part[t] : 0u;
+        uint32_t inc = wave_inclusive_add(x);
+        if (lane == kWave - 1) wtmp[w] = inc;
+        __syncthreads();
+        uint32_t before = 0, all = 0;
+        const int nw = (L + kWave - 1) / kWave;
+        for (int q = 0; q < nw; ++q) {
+            uint32_t y = wtmp[q];
+            before += q < w ? y : 0u;
+            all += y;
+        }
+        if (t < L) part[t] = before + inc - x;
+        __syncthreads();
+        total = all;
+    }
+    return total;
+}
+
+// MATRIX: EDPP_MATRIX_* ; the whole tile is n = L*tile elements
+template <int MATRIX>
+__global__ __launch_bounds__(kMaxLocal) void k_matrix_scan(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                           int tile, int network) {
+    extern __shared__ uint32_t lds[];
+    const int L = blockDim.x, t = threadIdx.x;
+    const int n = L * tile;
+    uint32_t* part = lds;          // [L]
+    uint32_t* wtmp = part + L;     // [16]
+    uint32_t* data = wtmp + 16;    // [n] for the LDS-staged variants
+    if (MATRIX == EDPP_MATRIX_LM_SERIAL) {
+        if (t == 0) {
+            uint32_t acc = 0;
+            for (int i = 0; i < n; ++i) {
+                uint32_t x = in[i];
+                out[i] = acc;
+                acc += x;
+            }
+        }
+        return;
+    }
+    uint32_t reg[kMaxRow];
+    uint32_t s = 0;
+    if (MATRIX == EDPP_MATRIX_REG) {
+        // row straight from global memory (lane-strided, uncoalesced)
+        #pragma unroll
+        for (int k = 0; k < kMaxRow; ++k)
+            if (k < tile) { reg[k] = in[t * tile + k]; s += reg[k]; }
+    } else {
+        // coalesced copy into LDS, then rows out of LDS
+        for (int i = t; i < n; i += L) data[i] = in[i];
+        __syncthreads();
+        #pragma unroll
+        for (int k = 0; k < kMaxRow; ++k)
+            if (k < tile) { reg[k] = data[t * tile + k]; s += reg[k]; }
+    }
+    part[t] = s;
+    __syncthreads();
+    scan_partials_network(part, L, network, wtmp);
+    uint32_t acc = part[t];
+    if (MATRIX == EDPP_MATRIX_LM_REG) {
+        __syncthreads();
+        #pragma unroll
+        for (int k = 0; k < kMaxRow; ++k)
+            if (k < tile) { data[t * tile + k] = acc; acc += reg[k]; }
+        __syncthreads();
+        for (int i = t; i < n; i += L) out[i] = data[i];
+    } else {
+        // rows written from registers (the reference's LM and REG final scans)
+        #pragma unroll
+        for (int k = 0; k < kMaxRow; ++k)
+            if (k < tile) { out[t * tile + k] = acc; acc += reg[k]; }
+    }
+}
+
+int launch_scan_local(edpp_ctx* c, const int32_t* in, int32_t* out, int tile, int local_size, int matrix_type,
+                      int network) {
+    if (tile < 1 || tile > kMaxRow)
+        return fail(c, EDPP_EINVAL, "tile_size %d outside [1,%d]", tile, kMaxRow);
+    if (local_size < kWave || local_size > kMaxLocal || (local_size & (local_size - 1)))
+        return fail(c, EDPP_EINVAL, "local_size %d must be a power of two in [64,1024]", local_size);
+    if (network < EDPP_NET_SERIAL || network > EDPP_NET_WAVE)
+        return fail(c, EDPP_EINVAL, "unknown network %d", network);
+    const size_t n = (size_t)local_size * tile;
+    size_t lds = (size_t)(local_size + 16) * 4;
+    if (matrix_type == EDPP_MATRIX_LM || matrix_type == EDPP_MATRIX_LM_REG) lds += n * 4;
+    if (lds > 160 * 1024) return fail(c, EDPP_EINVAL, "tile of %zu elements does not fit in LDS", n);
+    const uint32_t* a = (const uint32_t*)in;
+    uint32_t* b = (uint32_t*)out;
+    switch (matrix_type) {
+        case EDPP_MATRIX_LM:
+            EDPP_HIP(c, hipFuncSetAttribute((const void*)k_matrix_scan<EDPP_MATRIX_LM>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL(k_matrix_scan<EDPP_MATRIX_LM>, dim3(1), dim3(local_size), lds, c->stream, a, b, tile,
+                               network);
+            break;
+        case EDPP_MATRIX_REG:
+            hipLaunchKernelGGL(k_matrix_scan<EDPP_MATRIX_REG>, dim3(1), dim3(local_size), lds, c->stream, a, b, tile,
+                               network);
+            break;
+        case EDPP_MATRIX_LM_REG:
+            EDPP_HIP(c, hipFuncSetAttribute((const void*)k_matrix_scan<EDPP_MATRIX_LM_REG>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL(k_matrix_scan<EDPP_MATRIX_LM_REG>, dim3(1), dim3(local_size), lds, c->stream, a, b,
+                               tile, network);
+            break;
+        case EDPP_MATRIX_LM_SERIAL:
+            hipLaunchKernelGGL(k_matrix_scan<EDPP_MATRIX_LM_SERIAL>, dim3(1), dim3(local_size), lds, c->stream, a, b,
+                               tile, network);
+            break;
+        default:
+            return fail(c, EDPP_EINVAL, "unknown matrix scan type %d", matrix_type);
+    }
+    EDPP_HIP(c, hipGetLastError());
+    return EDPP_OK;
+}
+
+}  // namespace edpp

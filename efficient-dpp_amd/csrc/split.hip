@@ -1,0 +1,475 @@
+// split.hip — stable multi-split: bucket = key & (B-1), input order kept
+// inside every bucket, optional bucket start offsets.
+//
+// Reference: opencl/primitives/splitImpl.cpp:21-498 (WI / WG / single host
+// code), opencl/kernels/split_kernel.cl:320-801 (kernels),
+// cuda/test_split.cu:18-88 (warp-ballot multisplit).  The reference's fast
+// GPU splits rank with LDS atomic_inc and are therefore not stable
+// (split_kernel.cl:445,488,543); here every rank comes from ballot match
+// masks, so the output is the sequential stable partition.
+//
+// Strategies (all three produce identical output):
+//   WG      per-tile LDS histograms -> exclusive scan of the bucket-major
+//           [B][tiles] matrix -> per-tile scatter through an LDS reorder
+//           buffer so bucket runs leave the workgroup as contiguous stores
+//           (the reference's WG_split with VARIED_REORDER).
+//   WI      the same with one wave per histogram column and stores straight
+//           from registers (WI_split / the CUDA warp multisplit).
+//   SINGLE  one lane walks a contiguous chunk (single_split).
+#include <algorithm>
+#include "ctx.hpp"
+#include "device.hpp"
+
+namespace edpp {
+
+constexpr int kSplitBlock = 256;
+constexpr int kSplitWaves = kSplitBlock / kWave;
+
+// Mask of the lanes whose bucket equals the caller's, among valid lanes.
+__device__ __forceinline__ uint64_t match_bucket(uint32_t b, int lg, uint64_t valid) {
+    uint64_t m = valid;
+    for (int bit = 0; bit < lg; ++bit) {
+        bool on = (b >> bit) & 1u;
+        uint64_t bal = ballot(on);
+        m &= on ? bal : ~bal;
+    }
+    return m;
+}
+
+// XCD-aware tile order: workgroups are dispatched round-robin over the XCDs,
+// so workgroup g runs on XCD g % xcds.  Give every XCD a contiguous range of
+// tiles, so that the write frontier of adjacent tiles (adjacent runs of the
+// same bucket) meets in one L2.
+__device__ __forceinline__ int64_t xcd_tile(int64_t g, int64_t tiles, int xcds) {
+    if (xcds <= 1 || tiles < xcds) return g;
+    int64_t per = tiles / xcds, rem = tiles % xcds;  // first `rem` XCDs get per+1 tiles
+    int64_t x = g % xcds, k = g / xcds;
+    // rounds where every XCD still has a tile map by the split; the ragged last
+    // round (k == per) only exists for XCDs x < rem
+    int64_t start = x * per + (x < rem ? x : rem);
+    return start + k;
+}
+// LDS layout of the WG scatter: whist[waves][B], run[B], sub[B], off[B],
+// wtmp[16], then the key and value reorder buffers from a 16-byte boundary
+__host__ __device__ __forceinline__ int kbuf_word_offset(int B) {
+    return (((kSplitWaves + 3) * B + 16) + 3) & ~3;
+}
+
+// ------------------------------------------------------------- WG strategy
+// Tile = SUBS sub-tiles of S = 256*IPT elements.  In a sub-tile, wave w owns
+// [w*64*IPT, (w+1)*64*IPT) and item j of lane l is element w*64*IPT + j*64 + l,
+// so ranks follow input order: sub-tile, wave, item, lane.
+
+struct SplitGeom {
+    int64_t n;
+    int lg;          // log2 B
+    int64_t tile;    // elements per tile
+    int64_t tiles;
+    int xcds;
+};
+
+// per-wave counts of one sub-tile into whist[w][b] (zeroed by the caller)
+template <class K, int IPT>
+__device__ __forceinline__ void rank_subtile(const K* __restrict__ kin, int64_t sbase, int64_t n, int lg,
+                                             uint32_t* whist_w, uint32_t (&bk)[IPT], uint32_t (&rk)[IPT],
+                                             K (&key)[IPT]) {
+    const int lane = lane_id();
+    const int w = threadIdx.x / kWave;
+    const uint32_t mask = (1u << lg) - 1u;
+    const int64_t wbase = sbase + (int64_t)w * kWave * IPT;
+    #pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        int64_t e = wbase + (int64_t)j * kWave + lane;
+        bool ok = e < n;
+        key[j] = ok ? kin[e] : K(0);
+        bk[j] = ok ? ((uint32_t)key[j] & mask) : 0xFFFFFFFFu;
+    }
+    #pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        bool ok = bk[j] != 0xFFFFFFFFu;
+        uint64_t valid = ballot(ok);
+        uint64_t m = match_bucket(bk[j], lg, valid);
+        uint32_t below = count_below(m);
+        uint32_t cnt = (uint32_t)__popcll(m);
+        uint32_t prior = ok ? whist_w[bk[j]] : 0u;
+        rk[j] = prior + below;
+        // the highest lane of each group updates the count; the read above
+        // precedes this write in the wave's LDS instruction order
+        if (ok && below == cnt - 1) whist_w[bk[j]] = prior + cnt;
+    }
+}
+
+// exclusive scan over the B bucket totals held in cnt[] into off[] (block-wide)
+__device__ __forceinline__ void scan_buckets(const uint32_t* cnt, uint32_t* off, int B, uint32_t* wtmp) {
+    // each lane takes a run of per = ceil(B/256) consecutive buckets
+    const int per = (B + kSplitBlock - 1) / kSplitBlock;
+    const int b0 = threadIdx.x * per;
+    uint32_t s = 0;
+    for (int q = 0; q < per; ++q)
+        if (b0 + q < B) s += cnt[b0 + q];
+    uint32_t inc = wave_inclusive_add(s);
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == kWave - 1) wtmp[w] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int q = 0; q < kSplitWaves; ++q) before += q < w ? wtmp[q] : 0u;
+    uint32_t x = before + inc - s;
+    for (int q = 0; q < per; ++q)
+        if (b0 + q < B) {
+            uint32_t c = cnt[b0 + q];
+            off[b0 + q] = x;
+            x += c;
+        }
+}
+
+template <class K, int IPT>
+__global__ __launch_bounds__(kSplitBlock) void k_split_hist(const K* __restrict__ kin, SplitGeom g,
+                                                            int32_t* __restrict__ his) {
+    extern __shared__ uint32_t lds[];
+    const int B = 1 << g.lg;
+    uint32_t* whist = lds;                     // [waves][B]
+    constexpr int S = kSplitBlock * IPT;
+    const int64_t tile = xcd_tile(blockIdx.x, g.tiles, g.xcds);
+    const int64_t tbase = tile * g.tile;
+    const int64_t tend = std::min<int64_t>(tbase + g.tile, g.n);
+    const int w = threadIdx.x / kWave;
+    for (int b = threadIdx.x; b < kSplitWaves * B; b += kSplitBlock) whist[b] = 0;
+    __syncthreads();
+    for (int64_t s = tbase; s < tend; s += S) {
+        uint32_t bk[IPT], rk[IPT];
+        K key[IPT];
+        rank_subtile<K, IPT>(kin, s, tend, g.lg, whist + w * B, bk, rk, key);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < B; b += kSplitBlock) {
+        uint32_t a = 0;
+        for (int q = 0; q < kSplitWaves; ++q) a += whist[q * B + b];
+        his[(int64_t)b * g.tiles + tile] = (int32_t)a;
+    }
+}
+
+template <class K, int IPT, bool KV>
+__global__ __launch_bounds__(kSplitBlock) void k_split_scatter(const K* __restrict__ kin, K* __restrict__ kout,
+                                                               const int32_t* __restrict__ vin,
+                                                               int32_t* __restrict__ vout, SplitGeom g,
+                                                               const int32_t* __restrict__ scanned) {
+    extern __shared__ uint32_t lds[];
+    const int B = 1 << g.lg;
+    constexpr int S = kSplitBlock * IPT;
+    uint32_t* whist = lds;                       // [waves][B]
+    uint32_t* run = whist + kSplitWaves * B;     // [B] next output position of each bucket
+    uint32_t* sub = run + B;                     // [B] sub-tile bucket counts
+    uint32_t* off = sub + B;                     // [B] sub-tile bucket offsets
+    uint32_t* wtmp = off + B;                    // [waves]
+    K* kbuf = reinterpret_cast<K*>(lds + kbuf_word_offset(B));  // [S], 16-byte aligned
+    int32_t* vbuf = reinterpret_cast<int32_t*>(kbuf + S);
+    const uint32_t mask = (uint32_t)B - 1u;
+    const int64_t tile = xcd_tile(blockIdx.x, g.tiles, g.xcds);
+    const int64_t tbase = tile * g.tile;
+    const int64_t tend = std::min<int64_t>(tbase + g.tile, g.n);
+    const int w = threadIdx.x / kWave;
+    const int lane = lane_id();
+    for (int b = threadIdx.x; b < B; b += kSplitBlock) run[b] = (uint32_t)scanned[(int64_t)b * g.tiles + tile];
+    for (int64_t s = tbase; s < tend; s += S) {
+        for (int b = threadIdx.x; b < kSplitWaves * B; b += kSplitBlock) whist[b] = 0;
+        __syncthreads();
+        uint32_t bk[IPT], rk[IPT];
+        K key[IPT];
+        rank_subtile<K, IPT>(kin, s, tend, g.lg, whist + w * B, bk, rk, key);
+        int32_t val[IPT];
+        if (KV) {
+            const int64_t wbase = s + (int64_t)w * kWave * IPT;
+            #pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                int64_t e = wbase + (int64_t)j * kWave + lane;
+                val[j] = e < tend ? vin[e] : 0;
+            }
+        }
+        __syncthreads();
+        // wave prefixes per bucket (in place) and sub-tile totals
+        for (int b = threadIdx.x; b < B; b += kSplitBlock) {
+            uint32_t a = 0;
+            for (int q = 0; q < kSplitWaves; ++q) {
+                uint32_t x = whist[q * B + b];
+                whist[q * B + b] = a;
+                a += x;
+            }
+            sub[b] = a;
+        }
+        __syncthreads();
+        scan_buckets(sub, off, B, wtmp);
+        __syncthreads();
+        // place every element at its stable position inside the sub-tile
+        #pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            if (bk[j] != 0xFFFFFFFFu) {
+                uint32_t p = off[bk[j]] + whist[w * B + bk[j]] + rk[j];
+                if (p < (uint32_t)S) {  // always true for consistent ranks
+                    kbuf[p] = key[j];
+                    if (KV) vbuf[p] = val[j];
+                }
+            }
+        }
+        __syncthreads();
+        // stream the sub-tile out: consecutive slots of one bucket go to
+        // consecutive output positions
+        const int cnt = (int)std::min<int64_t>((int64_t)S, tend - s);
+        for (int i = threadIdx.x; i < cnt; i += kSplitBlock) {
+            K k = kbuf[i];
+            uint32_t b = (uint32_t)k & mask;
+            uint32_t pos = run[b] + ((uint32_t)i - off[b]);
+            if (pos < (uint64_t)g.n) {  // always true for a consistent histogram scan
+                kout[pos] = k;
+                if (KV) vout[pos] = vbuf[i];
+            }
+        }
+        __syncthreads();
+        for (int b = threadIdx.x; b < B; b += kSplitBlock) run[b] += sub[b];
+        // the next iteration's first barrier orders these updates before use
+    }
+}
+
+// d_start[b] = scanned[b*tiles] (opencl/kernels/split_kernel.cl:320-332)
+__global__ void k_split_start(const int32_t* scanned, int64_t tiles, int B, int32_t* d_start) {
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < B; b += gridDim.x * blockDim.x)
+        d_start[b] = scanned[(int64_t)b * tiles];
+}
+
+// ------------------------------------------------------------- WI strategy
+// One column of the histogram per wave chunk of 64*IPT elements.
+template <class K, int IPT>
+__global__ __launch_bounds__(kSplitBlock) void k_split_wave_hist(const K* __restrict__ kin, int64_t n, int lg,
+                                                                 int64_t cols, int32_t* __restrict__ his) {
+    extern __shared__ uint32_t lds[];
+    const int B = 1 << lg;
+    const int w = threadIdx.x / kWave;
+    uint32_t* h = lds + w * B;
+    const int64_t col = (int64_t)blockIdx.x * kSplitWaves + w;
+    for (int b = lane_id(); b < B; b += kWave) h[b] = 0;
+    __builtin_amdgcn_wave_barrier();
+    if (col < cols) {
+        uint32_t bk[IPT], rk[IPT];
+        K key[IPT];
+        // rank_subtile addresses the wave's chunk as sbase + w*64*IPT
+        const int64_t sbase = col * kWave * IPT - (int64_t)w * kWave * IPT;
+        rank_subtile<K, IPT>(kin, sbase, n, lg, h, bk, rk, key);
+        __builtin_amdgcn_wave_barrier();
+        for (int b = lane_id(); b < B; b += kWave) his[(int64_t)b * cols + col] = (int32_t)h[b];
+    }
+}
+
+template <class K, int IPT, bool KV>
+__global__ __launch_bounds__(kSplitBlock) void k_split_wave_scatter(const K* __restrict__ kin, K* __restrict__ kout,
+                                                                    const int32_t* __restrict__ vin,
+                                                                    int32_t* __restrict__ vout, int64_t n, int lg,
+                                                                    int64_t cols,
+                                                                    const int32_t* __restrict__ scanned) {
+    extern __shared__ uint32_t lds[];
+    const int B = 1 << lg;
+    const int w = threadIdx.x / kWave;
+    const int lane = lane_id();
+    uint32_t* h = lds + w * B;
+    const int64_t col = (int64_t)blockIdx.x * kSplitWaves + w;
+    for (int b = lane; b < B; b += kWave) h[b] = 0;
+    __builtin_amdgcn_wave_barrier();
+    if (col >= cols) return;
+    uint32_t bk[IPT], rk[IPT];
+    K key[IPT];
+    const int64_t sbase = col * kWave * IPT - (int64_t)w * kWave * IPT;
+    rank_subtile<K, IPT>(kin, sbase, n, lg, h, bk, rk, key);
+    const int64_t cbase = col * kWave * IPT;
+    #pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        if (bk[j] != 0xFFFFFFFFu) {
+            int64_t e = cbase + (int64_t)j * kWave + lane;
+            uint32_t pos = (uint32_t)scanned[(int64_t)bk[j] * cols + col] + rk[j];
+            if (pos < (uint64_t)n) {
+                kout[pos] = key[j];
+                if (KV) vout[pos] = vin[e];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------- SINGLE strategy
+// Lane g owns [g*len, (g+1)*len) and keeps its histogram column his[b][g].
+template <class K>
+__global__ void k_split_single_hist(const K* kin, int64_t n, int lg, int64_t len, int64_t lanes, int32_t* his) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= lanes) return;
+    const int B = 1 << lg;
+    for (int b = 0; b < B; ++b) his[(int64_t)b * lanes + g] = 0;
+    const int64_t e0 = g * len, e1 = std::min<int64_t>(e0 + len, n);
+    for (int64_t e = e0; e < e1; ++e) his[(int64_t)((uint32_t)kin[e] & (B - 1)) * lanes + g] += 1;
+}
+template <class K, bool KV>
+__global__ void k_split_single_scatter(const K* kin, K* kout, const int32_t* vin, int32_t* vout, int64_t n, int lg,
+                                       int64_t len, int64_t lanes, int32_t* scanned) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= lanes) return;
+    const int B = 1 << lg;
+    const int64_t e0 = g * len, e1 = std::min<int64_t>(e0 + len, n);
+    for (int64_t e = e0; e < e1; ++e) {
+        K k = kin[e];
+        int32_t* slot = scanned + (int64_t)((uint32_t)k & (B - 1)) * lanes + g;
+        int32_t p = *slot;
+        *slot = p + 1;
+        if ((uint32_t)p < (uint64_t)n) {
+            kout[p] = k;
+            if (KV) vout[p] = vin[e];
+        }
+    }
+}
+
+// ------------------------------------------------------------- host side
+namespace {
+
+template <int IPT, class K>
+size_t wg_lds_bytes(int B) {
+    return (size_t)kbuf_word_offset(B) * 4 + (size_t)kSplitBlock * IPT * (sizeof(K) + 4);
+}
+
+template <class K, int IPT>
+int run_wg(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t* vout, int32_t* d_start, int64_t n,
+           int lg) {
+    const int B = 1 << lg;
+    constexpr int S = kSplitBlock * IPT;
+    // tiles long enough that the [B][tiles] matrix stays small next to the data:
+    // at least 64 elements per bucket per tile on average
+    int64_t subs = std::max<int64_t>(1, ((int64_t)64 * B + S - 1) / S);
+    SplitGeom g;
+    g.n = n;
+    g.lg = lg;
+    g.tile = subs * S;
+    g.tiles = ceil_div(n, g.tile);
+    g.xcds = c->xcds;
+    const int64_t hlen = (int64_t)B * g.tiles;
+    if (hlen >= (1ll << 31)) return fail(c, EDPP_EUNSUP, "histogram of %lld entries too large", (long long)hlen);
+    void* w;
+    int rc = ws_reserve(c, (size_t)hlen * 4 + 256, &w);
+    if (rc) return rc;
+    int32_t* his = static_cast<int32_t*>(w);
+    size_t lh = (size_t)(kSplitWaves + 1) * B * 4;
+    size_t ls = wg_lds_bytes<IPT, K>(B);
+    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_hist<K, IPT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lh));
+    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_scatter<K, IPT, true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls));
+    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_scatter<K, IPT, false>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls));
+    hipLaunchKernelGGL((k_split_hist<K, IPT>), dim3((unsigned)g.tiles), dim3(kSplitBlock), lh, c->stream, kin, g, his);
+    EDPP_HIP(c, hipGetLastError());
+    rc = launch_scan<uint32_t>(c, (const uint32_t*)his, (uint32_t*)his, hlen, EDPP_SCAN_LOOKBACK, nullptr);
+    if (rc) return rc;
+    if (d_start)
+        hipLaunchKernelGGL(k_split_start, dim3((unsigned)ceil_div(B, 256)), dim3(256), 0, c->stream,
+                           (const int32_t*)his, g.tiles, B, d_start);
+    if (vin)
+        hipLaunchKernelGGL((k_split_scatter<K, IPT, true>), dim3((unsigned)g.tiles), dim3(kSplitBlock), ls, c->stream,
+                           kin, kout, vin, vout, g, (const int32_t*)his);
+    else
+        hipLaunchKernelGGL((k_split_scatter<K, IPT, false>), dim3((unsigned)g.tiles), dim3(kSplitBlock), ls,
+                           c->stream, kin, kout, vin, vout, g, (const int32_t*)his);
+    EDPP_HIP(c, hipGetLastError());
+    return EDPP_OK;
+}
+
+template <class K, int IPT>
+int run_wi(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t* vout, int32_t* d_start, int64_t n,
+           int lg) {
+    const int B = 1 << lg;
+    const int64_t cols = ceil_div(n, (int64_t)kWave * IPT);
+    const int64_t hlen = (int64_t)B * cols;
+    if (hlen >= (1ll << 31)) return fail(c, EDPP_EUNSUP, "histogram of %lld entries too large", (long long)hlen);
+    void* w;
+    int rc = ws_reserve(c, (size_t)hlen * 4 + 256, &w);
+    if (rc) return rc;
+    int32_t* his = static_cast<int32_t*>(w);
+    const unsigned grid = (unsigned)ceil_div(cols, kSplitWaves);
+    const size_t lds = (size_t)kSplitWaves * B * 4;
+    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_wave_hist<K, IPT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_wave_scatter<K, IPT, true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_wave_scatter<K, IPT, false>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_split_wave_hist<K, IPT>), dim3(grid), dim3(kSplitBlock), lds, c->stream, kin, n, lg, cols,
+                       his);
+    EDPP_HIP(c, hipGetLastError());
+    rc = launch_scan<uint32_t>(c, (const uint32_t*)his, (uint32_t*)his, hlen, EDPP_SCAN_LOOKBACK, nullptr);
+    if (rc) return rc;
+    if (d_start)
+        hipLaunchKernelGGL(k_split_start, dim3((unsigned)ceil_div(B, 256)), dim3(256), 0, c->stream,
+                           (const int32_t*)his, cols, B, d_start);
+    if (vin)
+        hipLaunchKernelGGL((k_split_wave_scatter<K, IPT, true>), dim3(grid), dim3(kSplitBlock), lds, c->stream, kin,
+                           kout, vin, vout, n, lg, cols, (const int32_t*)his);
+    else
+        hipLaunchKernelGGL((k_split_wave_scatter<K, IPT, false>), dim3(grid), dim3(kSplitBlock), lds, c->stream, kin,
+                           kout, vin, vout, n, lg, cols, (const int32_t*)his);
+    EDPP_HIP(c, hipGetLastError());
+    return EDPP_OK;
+}
+
+template <class K>
+int run_single(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t* vout, int32_t* d_start, int64_t n,
+               int lg) {
+    const int B = 1 << lg;
+    int64_t lanes = std::min<int64_t>((int64_t)c->cus * 64, std::max<int64_t>(1, n / 64));
+    while (lanes > 1 && (int64_t)B * lanes >= (1ll << 28)) lanes /= 2;
+    const int64_t len = ceil_div(n, lanes);
+    const int64_t hlen = (int64_t)B * lanes;
+    void* w;
+    int rc = ws_reserve(c, (size_t)hlen * 4 + 256, &w);
+    if (rc) return rc;
+    int32_t* his = static_cast<int32_t*>(w);
+    const unsigned grid = (unsigned)ceil_div(lanes, 64);
+    hipLaunchKernelGGL((k_split_single_hist<K>), dim3(grid), dim3(64), 0, c->stream, kin, n, lg, len, lanes, his);
+    EDPP_HIP(c, hipGetLastError());
+    rc = launch_scan<uint32_t>(c, (const uint32_t*)his, (uint32_t*)his, hlen, EDPP_SCAN_LOOKBACK, nullptr);
+    if (rc) return rc;
+    if (d_start)
+        hipLaunchKernelGGL(k_split_start, dim3((unsigned)ceil_div(B, 256)), dim3(256), 0, c->stream,
+                           (const int32_t*)his, lanes, B, d_start);
+    if (vin)
+        hipLaunchKernelGGL((k_split_single_scatter<K, true>), dim3(grid), dim3(64), 0, c->stream, kin, kout, vin, vout,
+                           n, lg, len, lanes, his);
+    else
+        hipLaunchKernelGGL((k_split_single_scatter<K, false>), dim3(grid), dim3(64), 0, c->stream, kin, kout, vin,
+                           vout, n, lg, len, lanes, his);
+    EDPP_HIP(c, hipGetLastError());
+    return EDPP_OK;
+}
+
+}  // namespace
+
+template <class K>
+int launch_split(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t* vout, int32_t* d_start, int64_t n,
+                 int buckets, int strategy) {
+    int lg = 0;
+    while ((1 << lg) < buckets) ++lg;
+    if (n == 0) {
+        if (d_start) EDPP_HIP(c, hipMemsetAsync(d_start, 0, sizeof(int32_t) * buckets, c->stream));
+        return EDPP_OK;
+    }
+    if (strategy == EDPP_SPLIT_AUTO) strategy = default_split_strategy(c, n, buckets);
+    switch (strategy) {
+        case EDPP_SPLIT_WG:
+            // keep the LDS footprint of the scatter under 160 KiB
+            if (wg_lds_bytes<16, K>(buckets) <= 160 * 1024) return run_wg<K, 16>(c, kin, kout, vin, vout, d_start, n, lg);
+            return run_wg<K, 8>(c, kin, kout, vin, vout, d_start, n, lg);
+        case EDPP_SPLIT_WI:
+            return run_wi<K, 16>(c, kin, kout, vin, vout, d_start, n, lg);
+        case EDPP_SPLIT_SINGLE:
+            return run_single<K>(c, kin, kout, vin, vout, d_start, n, lg);
+        default:
+            return fail(c, EDPP_EINVAL, "unknown split strategy %d", strategy);
+    }
+}
+
+template int launch_split<uint32_t>(edpp_ctx*, const uint32_t*, uint32_t*, const int32_t*, int32_t*, int32_t*, int64_t,
+                                    int, int);
+template int launch_split<uint64_t>(edpp_ctx*, const uint64_t*, uint64_t*, const int32_t*, int32_t*, int32_t*,
+                                    int64_t, int, int);
+
+}  // namespace edpp

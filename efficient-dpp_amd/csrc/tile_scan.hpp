@@ -1,0 +1,203 @@
+// tile_scan.hpp — workgroup tile scan and the decoupled look-back protocol.
+//
+// Tile layout: a tile of TILE = BLOCK*EPV*VPT elements is VPT sub-tiles of
+// BLOCK*EPV contiguous elements.  In sub-tile j lane t owns EPV consecutive
+// elements, loaded with one 16-byte (dwordx4) access, so every load and store
+// is coalesced and no LDS transpose is needed.  Prefix order inside the tile:
+// sub-tile j, then wave w, then lane, then element.
+#pragma once
+#include "device.hpp"
+#include "edpp.h"
+
+namespace edpp {
+
+enum : uint32_t { kFlagNone = 0, kFlagAggregate = 1, kFlagPrefix = 2 };
+
+template <class T, int BLOCK, int VPT>
+struct TileScan {
+    typedef typename vec16<T>::type V;
+    static constexpr int EPV = vec16<T>::n;
+    static constexpr int SUB = BLOCK * EPV;
+    static constexpr int TILE = SUB * VPT;
+    static constexpr int NW = BLOCK / kWave;
+    static_assert(BLOCK % kWave == 0, "block must be whole waves");
+
+    struct Smem {
+        T wave_tot[VPT][NW];
+        T prefix;
+        uint32_t tile;
+    };
+
+    T v[VPT][EPV];
+    T s[VPT];     // per-lane sum of sub-tile j
+    T off[VPT];   // exclusive offset of the lane's first element of sub-tile j, within the tile
+    T total;      // tile sum
+
+    __device__ __forceinline__ void load(const T* __restrict__ in, int64_t base, int64_t n) {
+        const int t = threadIdx.x;
+        if (base + TILE <= n) {
+            #pragma unroll
+            for (int j = 0; j < VPT; ++j) {
+                V x = ld_stream(reinterpret_cast<const V*>(in + base + (int64_t)j * SUB) + t);
+                #pragma unroll
+                for (int k = 0; k < EPV; ++k) v[j][k] = x[k];
+            }
+        } else {
+            #pragma unroll
+            for (int j = 0; j < VPT; ++j) {
+                #pragma unroll
+                for (int k = 0; k < EPV; ++k) {
+                    int64_t e = base + (int64_t)j * SUB + (int64_t)t * EPV + k;
+                    v[j][k] = e < n ? in[e] : T(0);
+                }
+            }
+        }
+    }
+
+    // Block-wide scan of the tile; contains one __syncthreads.
+    __device__ __forceinline__ void scan(Smem& sm) {
+        const int lane = lane_id();
+        const int w = threadIdx.x / kWave;
+        T inc[VPT];
+        #pragma unroll
+        for (int j = 0; j < VPT; ++j) {
+            T a = 0;
+            #pragma unroll
+            for (int k = 0; k < EPV; ++k) a += v[j][k];
+            s[j] = a;
+            inc[j] = wave_inclusive_add(a);
+            if (lane == kWave - 1) sm.wave_tot[j][w] = inc[j];
+        }
+        __syncthreads();
+        T run = 0;
+        #pragma unroll
+        for (int j = 0; j < VPT; ++j) {
+            T before = 0, sub = 0;
+            #pragma unroll
+            for (int q = 0; q < NW; ++q) {
+                T x = sm.wave_tot[j][q];
+                before += (q < w) ? x : T(0);
+                sub += x;
+            }
+            off[j] = run + before + inc[j] - s[j];
+            run += sub;
+        }
+        total = run;
+    }
+
+    __device__ __forceinline__ void store(T* __restrict__ out, int64_t base, int64_t n, T prefix) const {
+        const int t = threadIdx.x;
+        const bool full = base + TILE <= n;
+        #pragma unroll
+        for (int j = 0; j < VPT; ++j) {
+            T x = prefix + off[j];
+            V o;
+            #pragma unroll
+            for (int k = 0; k < EPV; ++k) { o[k] = x; x += v[j][k]; }
+            if (full) {
+                st_stream(o, reinterpret_cast<V*>(out + base + (int64_t)j * SUB) + t);
+            } else {
+                #pragma unroll
+                for (int k = 0; k < EPV; ++k) {
+                    int64_t e = base + (int64_t)j * SUB + (int64_t)t * EPV + k;
+                    if (e < n) out[e] = o[k];
+                }
+            }
+        }
+    }
+};
+
+// Block-wide reduction helper (one __syncthreads); result valid in all lanes.
+template <class T, int BLOCK>
+__device__ __forceinline__ T block_sum(T x, T* lds /* BLOCK/64 entries */) {
+    constexpr int NW = BLOCK / kWave;
+    T ws = wave_sum(x);
+    if (lane_id() == 0) lds[threadIdx.x / kWave] = ws;
+    __syncthreads();
+    T r = 0;
+    #pragma unroll
+    for (int q = 0; q < NW; ++q) r += lds[q];
+    return r;
+}
+
+// ---------------------------------------------------------------- look-back state
+// 32-bit values: one 64-bit word per tile = {epoch:30 | flag:2 | value:32}, so a
+// single relaxed agent-scope load observes flag and value together.
+struct LookbackState32 {
+    uint64_t* status;
+    uint32_t epoch;
+    __device__ __forceinline__ void publish(int64_t t, uint32_t flag, uint32_t value) const {
+        uint64_t w = ((uint64_t)epoch << 34) | ((uint64_t)flag << 32) | value;
+        st_relaxed(status + t, w);
+    }
+    __device__ __forceinline__ void read(int64_t t, uint32_t& flag, uint32_t& value) const {
+        uint64_t w = ld_relaxed(status + t);
+        bool live = (uint32_t)(w >> 34) == epoch;
+        flag = live ? (uint32_t)(w >> 32) & 3u : kFlagNone;
+        value = (uint32_t)w;
+    }
+};
+
+// 64-bit values: a flag word {epoch:30 | flag:2} per tile published with release
+// after the value, which sits in a separate slot per flag kind (no torn reads).
+struct LookbackState64 {
+    uint32_t* flags;
+    uint64_t* vals;  // [2*t] aggregate, [2*t+1] inclusive prefix
+    uint32_t epoch;
+    __device__ __forceinline__ void publish(int64_t t, uint32_t flag, uint64_t value) const {
+        st_relaxed(vals + 2 * t + (flag == kFlagPrefix ? 1 : 0), value);
+        st_release(flags + t, (epoch << 2) | flag);
+    }
+    __device__ __forceinline__ void read(int64_t t, uint32_t& flag, uint64_t& value) const {
+        uint32_t f = ld_acquire(flags + t);
+        flag = (f >> 2) == epoch ? (f & 3u) : kFlagNone;
+        value = flag == kFlagNone ? 0 : ld_relaxed(vals + 2 * t + (flag == kFlagPrefix ? 1 : 0));
+    }
+};
+
+template <class T> struct LookbackFor;
+template <> struct LookbackFor<uint32_t> { typedef LookbackState32 type; };
+template <> struct LookbackFor<uint64_t> { typedef LookbackState64 type; };
+
+// Decoupled look-back, executed by one whole wave for tile `tile` > 0.  Lane l
+// inspects predecessor (cursor - l); the window advances by 64 until a tile
+// with an inclusive prefix is found.  Waiting only on smaller tickets, which
+// belong to workgroups that are already running, guarantees progress.
+// Returns the exclusive prefix (valid in every lane).  On watchdog expiry
+// sets *err and returns what it has (the caller's result is then invalid and
+// the host reports EDPP_ETIMEOUT).
+template <class T, class State>
+__device__ __forceinline__ T lookback(const State& st, int64_t tile, uint32_t spin_limit, int32_t* err) {
+    const int lane = lane_id();
+    T excl = 0;
+    int64_t cursor = tile - 1;
+    uint32_t spins = 0;
+    while (true) {
+        int64_t p = cursor - lane;
+        uint32_t flag;
+        T val;
+        if (p >= 0) {
+            st.read(p, flag, val);
+        } else {
+            flag = kFlagPrefix;
+            val = 0;
+        }
+        uint64_t pm = ballot(flag == kFlagPrefix);
+        uint64_t zm = ballot(flag == kFlagNone);
+        uint64_t upto = pm ? (pm ^ (pm - 1)) : ~0ull;  // lanes 0..first prefix lane
+        if (zm & upto) {
+            if (++spins > spin_limit) {
+                if (lane == 0) st_relaxed(err, (int32_t)EDPP_ETIMEOUT);
+                return excl;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        T c = ((upto >> lane) & 1ull) ? val : T(0);
+        excl += wave_sum(c);
+        if (pm) return excl;
+        cursor -= kWave;
+    }
+}
+
+}  // namespace edpp

@@ -1,0 +1,186 @@
+/*
+ * edpp.h — C ABI of the MI355X (gfx950) data-parallel primitives library.
+ *
+ * Every entry point takes device pointers that the caller allocated, plain
+ * sizes and enums, and returns an int status (EDPP_OK == 0).  No C++ types,
+ * no exceptions and no torch types cross this boundary.  Work is enqueued on
+ * the context's HIP stream; when `ms` is non-NULL the call synchronises and
+ * stores the summed device time of its kernels in milliseconds, which is the
+ * value the reference's primitives return (opencl/util/utility.cpp:99-104,
+ * opencl/primitives/gatherImpl.cpp:52).
+ *
+ * Reference interfaces replaced (file:line in RapidsAtHKUST/Efficient-DPP):
+ *   gather            opencl/primitives.h:12-14   -> edpp_gather_i32
+ *   scatter           opencl/primitives.h:17-19   -> edpp_scatter_i32
+ *   scan_chained      opencl/primitives.h:22-24   -> edpp_scan_exclusive_i32 (EDPP_SCAN_LOOKBACK)
+ *   scan_RSS          opencl/primitives.h:26-27   -> edpp_scan_exclusive_i32 (EDPP_SCAN_RTS)
+ *   scan_RSS_single   opencl/primitives.h:29      -> edpp_scan_exclusive_i32 (EDPP_SCAN_SSA)
+ *   matrix scans      opencl/kernels/scan_local_kernel.cl:530-695,
+ *                     opencl/test/test_scan_local.cpp:6-106 -> edpp_scan_local_i32
+ *   WI_split          opencl/primitives.h:32-37   -> edpp_split_i32 (EDPP_SPLIT_WI)
+ *   WG_split          opencl/primitives.h:39-44   -> edpp_split_i32 (EDPP_SPLIT_WG*)
+ *   single_split      opencl/primitives.h:46-49   -> edpp_split_i32 (EDPP_SPLIT_SINGLE)
+ *   STREAM kernels    opencl/kernels/mem_kernel.cl:9-31 -> edpp_stream_i32
+ *   data generators   opencl/util/utility.cpp:214-241 -> edpp_gen_* (reproducible)
+ *
+ * All results are bit-exact with the sequential definitions:
+ *   scan:    out[i] = sum_{j<i} in[j]  (two's-complement wrap-around)
+ *   gather:  out[i] = in[idx[i]]       scatter: out[idx[i]] = in[i]
+ *   split:   stable counting sort by (key & (buckets-1)); start[b] = #keys in buckets < b
+ * and they do not depend on the schedule (scheme, pass count, tile) chosen.
+ */
+#ifndef EDPP_H
+#define EDPP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EDPP_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum edpp_status {
+    EDPP_OK       = 0,
+    EDPP_EINVAL   = -1,  /* bad argument (null pointer, n<0, n>=2^31, B not a power of two, ...) */
+    EDPP_ENOMEM   = -2,  /* workspace allocation failed                                            */
+    EDPP_ETIMEOUT = -3,  /* a bounded device wait ran out (look-back watchdog)                     */
+    EDPP_EHIP     = -4,  /* a HIP runtime call failed                                              */
+    EDPP_ENODEV   = -5,  /* no HIP device                                                           */
+    EDPP_EUNSUP   = -6   /* combination not supported                                               */
+};
+
+/* ---- enums mirroring opencl/types.h:8-46 -------------------------------- */
+enum edpp_scan_scheme {          /* SCAN_GLOBAL_TYPE, opencl/types.h:8-10 */
+    EDPP_SCAN_AUTO     = 0,
+    EDPP_SCAN_LOOKBACK = 1,      /* single pass, decoupled look-back (replaces CHAINED)            */
+    EDPP_SCAN_RTS      = 2,      /* reduce -> scan partials -> scan tiles with offsets (RSS)       */
+    EDPP_SCAN_SSA      = 3       /* scan tiles -> scan partials -> add offsets                     */
+};
+
+enum edpp_matrix_scan {          /* MatrixScanType, opencl/types.h:12-14 */
+    EDPP_MATRIX_LM        = 0,   /* tile staged in LDS, per-thread rows                           */
+    EDPP_MATRIX_REG       = 1,   /* rows loaded straight into registers (strided)                 */
+    EDPP_MATRIX_LM_REG    = 2,   /* coalesced load to LDS, rows to registers, back through LDS    */
+    EDPP_MATRIX_LM_SERIAL = 3    /* one lane scans the whole tile                                 */
+};
+
+enum edpp_local_network {        /* intra-workgroup scan of the per-thread partials,
+                                    opencl/kernels/scan_local_kernel.cl:42-259 */
+    EDPP_NET_SERIAL   = 0,       /* lane 0 walks the partials (the reference's active choice, :504) */
+    EDPP_NET_BLELLOCH = 1,       /* work-efficient up/down sweep in LDS                           */
+    EDPP_NET_SKLANSKY = 2,       /* log-depth Sklansky in LDS                                      */
+    EDPP_NET_WAVE     = 3        /* 64-lane DPP wave scan + LDS across waves                       */
+};
+
+enum edpp_split_strategy {       /* SPLIT_ALGO / ReorderType, opencl/types.h:32-42 */
+    EDPP_SPLIT_AUTO     = 0,
+    EDPP_SPLIT_WG       = 1,     /* per-tile histogram -> scan of [B][tiles] -> stable scatter, LDS reorder */
+    EDPP_SPLIT_WI       = 2,     /* per-wave histograms [B][waves] -> scan -> stable scatter, no reorder     */
+    EDPP_SPLIT_SINGLE   = 3      /* one lane per contiguous chunk (the reference's single_split)            */
+};
+
+enum edpp_stream_op {            /* opencl/kernels/mem_kernel.cl:9-31 */
+    EDPP_STREAM_COPY  = 0,       /* c[i] = a[i]          */
+    EDPP_STREAM_SCALE = 1,       /* c[i] = 3*a[i]        */
+    EDPP_STREAM_ADD   = 2,       /* c[i] = a[i] + b[i]   */
+    EDPP_STREAM_TRIAD = 3        /* c[i] = a[i] + 3*b[i] */
+};
+
+enum edpp_perm_kind {
+    EDPP_PERM_RANDOM      = 0,   /* pseudo-random permutation of [0,n) (keyed Feistel, cycle walking) */
+    EDPP_PERM_BLOCK_LOCAL = 1,   /* 4096-element blocks permuted, order inside each block permuted    */
+    EDPP_PERM_IDENTITY    = 2
+};
+
+enum edpp_key_dist {
+    EDPP_KEYS_UNIFORM = 0,       /* bucket uniform on [0,B)                   */
+    EDPP_KEYS_ZIPF    = 1        /* bucket ~ Zipf(s=1) over ranks 1..B        */
+};
+
+typedef struct edpp_ctx edpp_ctx;
+
+/* ---- context ------------------------------------------------------------
+ * One context per host thread.  `stream` may be NULL (the context creates a
+ * default-priority stream of its own) or an existing hipStream_t to enqueue
+ * on (for example the caller's framework stream).  The context owns a
+ * device workspace that grows on demand (hipMalloc outside the timed
+ * kernels, as the reference allocates its temporaries per call,
+ * opencl/primitives/scanImpl.cpp:39, splitImpl.cpp:76).                    */
+int  edpp_ctx_create(int device, void* stream, edpp_ctx** out);
+int  edpp_ctx_destroy(edpp_ctx* ctx);
+int  edpp_ctx_sync(edpp_ctx* ctx);
+void* edpp_ctx_stream(edpp_ctx* ctx);
+/* Number of compute units and XCDs the context's device reports. */
+int  edpp_ctx_device_info(edpp_ctx* ctx, int* cus, int* xcds, int64_t* l2_bytes, int64_t* mall_bytes);
+const char* edpp_strerror(int status);
+const char* edpp_last_error(edpp_ctx* ctx);   /* message of the last failing call on ctx */
+int  edpp_abi_version(void);
+
+/* ---- STREAM probe: opencl/kernels/mem_kernel.cl:9-31, cuda/test_bandwidth.cu:14-44 */
+int edpp_stream_i32(edpp_ctx* ctx, int op, const int32_t* a, const int32_t* b,
+                    int32_t* c, int64_t n, double* ms);
+
+/* ---- multi-pass gather / scatter ----------------------------------------
+ * opencl/primitives/gatherImpl.cpp:7-55, scatterImpl.cpp:9-54.  Pass k of
+ * `passes` touches only indices in [k*ceil(n/passes), (k+1)*ceil(n/passes)).
+ * passes == 0 picks the count from the context's schedule (tuner or model).
+ * idx must hold values in [0,n); for scatter it must be a permutation.      */
+int edpp_gather_i32 (edpp_ctx* ctx, const int32_t* in, int32_t* out, const int32_t* idx,
+                     int64_t n, int passes, double* ms);
+int edpp_scatter_i32(edpp_ctx* ctx, const int32_t* in, int32_t* out, const int32_t* idx,
+                     int64_t n, int passes, double* ms);
+int edpp_gather_i64 (edpp_ctx* ctx, const int64_t* in, int64_t* out, const int32_t* idx,
+                     int64_t n, int passes, double* ms);
+int edpp_scatter_i64(edpp_ctx* ctx, const int64_t* in, int64_t* out, const int32_t* idx,
+                     int64_t n, int passes, double* ms);
+
+/* ---- global exclusive scan ----------------------------------------------
+ * opencl/primitives/scanImpl.cpp:15-236.  in == out is allowed (the
+ * reference scans split histograms in place, splitImpl.cpp:253).
+ * d_total (device pointer, may be NULL) receives sum(in) mod 2^32 / 2^64.   */
+int edpp_scan_exclusive_i32(edpp_ctx* ctx, const int32_t* in, int32_t* out, int64_t n,
+                            int scheme, int32_t* d_total, double* ms);
+int edpp_scan_exclusive_i64(edpp_ctx* ctx, const int64_t* in, int64_t* out, int64_t n,
+                            int scheme, int64_t* d_total, double* ms);
+
+/* ---- single-workgroup tile scans ----------------------------------------
+ * opencl/kernels/scan_local_kernel.cl:478-695, driver
+ * opencl/test/test_scan_local.cpp:6-106: one workgroup of `local_size`
+ * lanes scans n = local_size*tile_size values, exclusive.                    */
+int edpp_scan_local_i32(edpp_ctx* ctx, const int32_t* in, int32_t* out, int tile_size,
+                        int local_size, int matrix_type, int network, double* ms);
+
+/* ---- stable multi-split --------------------------------------------------
+ * opencl/primitives/splitImpl.cpp:21-498, cuda/test_split.cu:18-88.
+ * bucket = key & (buckets-1); buckets is a power of two in [1, 4096].
+ * vals_in/vals_out may both be NULL (key-only, KO) or both non-NULL (SOA).
+ * d_bucket_start (may be NULL) receives int32[buckets] start offsets
+ * (the reference's gatherStartPos, opencl/kernels/split_kernel.cl:320-332).  */
+int edpp_split_i32(edpp_ctx* ctx, const int32_t* keys_in, int32_t* keys_out,
+                   const int32_t* vals_in, int32_t* vals_out, int32_t* d_bucket_start,
+                   int64_t n, int buckets, int strategy, double* ms);
+int edpp_split_i64(edpp_ctx* ctx, const int64_t* keys_in, int64_t* keys_out,
+                   const int32_t* vals_in, int32_t* vals_out, int32_t* d_bucket_start,
+                   int64_t n, int buckets, int strategy, double* ms);
+
+/* ---- reproducible synthetic inputs ---------------------------------------
+ * Replace opencl/util/utility.cpp:214-241 (time-seeded, schedule-dependent)
+ * with counter-based SplitMix64: x_i = i-th output of SplitMix64(seed).      */
+/* out[i] = (int32)(x_i >> 32) & mask  (mask = -1 for full-range values)     */
+int edpp_gen_i32(edpp_ctx* ctx, int32_t* out, int64_t n, uint64_t seed, int32_t mask);
+/* out[i] = (int64)x_i & mask */
+int edpp_gen_i64(edpp_ctx* ctx, int64_t* out, int64_t n, uint64_t seed, int64_t mask);
+int edpp_gen_perm(edpp_ctx* ctx, int32_t* idx, int64_t n, uint64_t seed, int kind);
+/* keys with bucket (key & (B-1)) drawn from `dist`; vals (may be NULL) = i   */
+int edpp_gen_split_keys_i32(edpp_ctx* ctx, int32_t* keys, int32_t* vals, int64_t n,
+                            int buckets, int dist, uint64_t seed);
+int edpp_gen_split_keys_i64(edpp_ctx* ctx, int64_t* keys, int32_t* vals, int64_t n,
+                            int buckets, int dist, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EDPP_H */

@@ -1,0 +1,300 @@
+"""GPU parity: every primitive through the C ABI against the CPU oracle on the
+same seeded inputs, bit-exact, plus size-independent properties at the full
+2^28 sizes."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [0, 1, 2, 3, 63, 64, 65, 1023, 1024, 1025, 4095, 4096, 4097, 65_537, 1 << 20, (1 << 20) + 17, 3_000_001]
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def empty(torch, n, dtype):
+    return torch.empty(max(n, 1), dtype=dtype, device="cuda")[:n]
+
+
+# ---------------------------------------------------------------- generators
+@pytest.mark.parametrize("n", [1, 1000, 4096 * 3 + 7, 1 << 20])
+def test_generators_match_oracle(torch_gpu, ctx, orc, n):
+    torch = torch_gpu
+    seed = 0x1234 + n
+    a = empty(torch, n, torch.int32)
+    ctx.gen_i32(a, seed)
+    assert np.array_equal(host(a), orc.gen_i32(n, seed))
+    ctx.gen_i32(a, seed, 15)
+    assert np.array_equal(host(a), orc.gen_i32(n, seed, 15))
+    b = empty(torch, n, torch.int64)
+    ctx.gen_i64(b, seed)
+    assert np.array_equal(host(b), orc.gen_i64(n, seed))
+    for kind in (0, 1, 2):
+        ctx.gen_perm(a, seed, kind)
+        assert np.array_equal(host(a), orc.gen_perm(n, seed, kind)), kind
+    v = empty(torch, n, torch.int32)
+    for B in (1, 2, 16, 256, 4096):
+        for dist in (0, 1):
+            ctx.gen_split_keys(a, v, B, dist, seed)
+            k, vv = orc.gen_split_keys(n, B, dist, seed)
+            assert np.array_equal(host(a), k) and np.array_equal(host(v), vv), (B, dist)
+            ctx.gen_split_keys(b, v, B, dist, seed, wide=True)
+            k, _ = orc.gen_split_keys(n, B, dist, seed, wide=True)
+            assert np.array_equal(host(b), k), (B, dist)
+
+
+# ---------------------------------------------------------------- STREAM
+@pytest.mark.parametrize("n", [0, 1, 5, 1027, 1 << 20, (1 << 20) + 3])
+def test_stream_ops(torch_gpu, ctx, orc, n):
+    torch = torch_gpu
+    a, b = orc.gen_i32(n, 1), orc.gen_i32(n, 2)
+    da, db = dev(torch, a), dev(torch, b)
+    c = empty(torch, n, torch.int32)
+    for op in range(4):
+        ctx.stream_op(op, da, db, c, n)
+        assert np.array_equal(host(c), orc.stream(op, a, b if op >= 2 else None)), op
+    # unaligned pointers take the scalar path
+    if n > 8:
+        ctx.stream_op(3, da[1:], db[1:], c[1:], n - 1)
+        assert np.array_equal(host(c)[1:], orc.stream(3, a[1:], b[1:]))
+
+
+# ---------------------------------------------------------------- scans
+@pytest.mark.parametrize("scheme", [1, 2, 3])
+@pytest.mark.parametrize("n", SIZES)
+def test_scan_i32(torch_gpu, ctx, orc, scheme, n):
+    torch = torch_gpu
+    for mask in (15, -1):
+        x = orc.gen_i32(n, 100 + n, mask)
+        ref, tot = orc.scan_exclusive(x)
+        dx = dev(torch, x)
+        out = empty(torch, n, torch.int32)
+        total = torch.full((1,), 7, dtype=torch.int32, device="cuda")
+        ctx.scan(dx, out, n, scheme=scheme, total=total)
+        assert np.array_equal(host(out), ref), (scheme, n, mask)
+        assert int(host(total)[0]) == int(tot)
+
+
+@pytest.mark.parametrize("scheme", [1, 2, 3])
+@pytest.mark.parametrize("n", [1, 4097, 1 << 20, 3_000_001])
+def test_scan_i64(torch_gpu, ctx, orc, scheme, n):
+    torch = torch_gpu
+    x = orc.gen_i64(n, 7 + n)
+    ref, tot = orc.scan_exclusive(x)
+    out = empty(torch, n, torch.int64)
+    total = torch.zeros(1, dtype=torch.int64, device="cuda")
+    ctx.scan(dev(torch, x), out, n, scheme=scheme, total=total, wide=True)
+    assert np.array_equal(host(out), ref)
+    assert int(host(total)[0]) == int(tot)
+
+
+@pytest.mark.parametrize("scheme", [1, 2, 3])
+def test_scan_in_place_and_unaligned(torch_gpu, ctx, orc, scheme):
+    torch = torch_gpu
+    n = 1_000_003
+    x = orc.gen_i32(n, 5)
+    ref, _ = orc.scan_exclusive(x)
+    d = dev(torch, x)
+    ctx.scan(d, d, n, scheme=scheme)  # in place, as split scans its histogram
+    assert np.array_equal(host(d), ref)
+    d = dev(torch, np.concatenate([[0], x]))
+    out = empty(torch, n + 1, torch.int32)
+    ctx.scan(d[1:], out[1:], n, scheme=scheme)  # 4-byte offset: not 16-byte aligned
+    assert np.array_equal(host(out)[1:], ref)
+
+
+def test_scan_c1_config(torch_gpu, ctx, orc):
+    # 2^24 int32 values uniform in [0,15] from a fixed seed, all three schemes
+    torch = torch_gpu
+    n = 1 << 24
+    x = orc.gen_i32(n, 2024, 15)
+    ref, tot = orc.scan_exclusive(x)
+    d = empty(torch, n, torch.int32)
+    ctx.gen_i32(d, 2024, 15)
+    out = empty(torch, n, torch.int32)
+    total = torch.zeros(1, dtype=torch.int32, device="cuda")
+    for scheme in (1, 2, 3):
+        out.fill_(-1)
+        ctx.scan(d, out, scheme=scheme, total=total)
+        assert np.array_equal(host(out), ref), scheme
+        assert int(host(total)[0]) == int(tot)
+
+
+def test_scan_full_size_properties(torch_gpu, ctx):
+    # 2^28 values in [0,15]: total wraps past 2^31; adjacent differences equal
+    # the input and the total equals the (wrapped) sum
+    torch = torch_gpu
+    n = 1 << 28
+    x = empty(torch, n, torch.int32)
+    ctx.gen_i32(x, 77, 15)
+    out = empty(torch, n, torch.int32)
+    total = torch.zeros(1, dtype=torch.int32, device="cuda")
+    for scheme in (1, 2, 3):
+        ctx.scan(x, out, scheme=scheme, total=total)
+        assert int(out[0]) == 0
+        assert torch.equal(out[1:] - out[:-1], x[:-1])  # int32 wrap-around subtraction
+        s = int(x.to(torch.int64).sum().item())
+        assert int(total.item()) % (1 << 32) == s % (1 << 32)
+        assert (int(out[-1].item()) + int(x[-1].item())) % (1 << 32) == s % (1 << 32)
+    del x, out
+
+
+# ---------------------------------------------------------------- local (matrix) scans
+@pytest.mark.parametrize("matrix", [0, 1, 2, 3])
+@pytest.mark.parametrize("network", [0, 1, 2, 3])
+@pytest.mark.parametrize("tile,local", [(1, 512), (11, 512), (49, 512), (64, 64), (16, 1024), (3, 128)])
+def test_scan_local(torch_gpu, ctx, orc, matrix, network, tile, local):
+    torch = torch_gpu
+    n = tile * local
+    x = orc.gen_i32(n, tile * 1000 + local, 15)
+    out = empty(torch, n, torch.int32)
+    ctx.scan_local(dev(torch, x), out, tile, local, matrix, network)
+    assert np.array_equal(host(out), orc.scan_exclusive(x)[0])
+
+
+# ---------------------------------------------------------------- gather / scatter
+@pytest.mark.parametrize("n", [1, 3, 4, 5, 1000, 4096 * 5 + 3, 1 << 20, 3_000_001])
+@pytest.mark.parametrize("kind", [0, 1])
+def test_gather_scatter(torch_gpu, ctx, orc, n, kind):
+    torch = torch_gpu
+    vals = orc.gen_i32(n, 3 + n)
+    idx = orc.gen_perm(n, 4 + n, kind)
+    dv, di = dev(torch, vals), dev(torch, idx)
+    out = empty(torch, n, torch.int32)
+    gref = vals[idx]
+    sref = np.empty_like(vals)
+    sref[idx] = vals
+    for passes in (1, 2, 3, 8, 16):
+        out.fill_(0)
+        ctx.gather(dv, out, di, n, passes)
+        assert np.array_equal(host(out), gref), ("gather", passes)
+        out.fill_(0)
+        ctx.scatter(dv, out, di, n, passes)
+        assert np.array_equal(host(out), sref), ("scatter", passes)
+
+
+@pytest.mark.parametrize("n", [7, 4096 * 3 + 1, 1 << 20])
+def test_gather_scatter_i64_and_unaligned(torch_gpu, ctx, orc, n):
+    torch = torch_gpu
+    vals = orc.gen_i64(n, 9)
+    idx = orc.gen_perm(n, 10, 0)
+    dv, di = dev(torch, vals), dev(torch, idx)
+    out = empty(torch, n, torch.int64)
+    for passes in (1, 4):
+        ctx.gather(dv, out, di, n, passes, wide=True)
+        assert np.array_equal(host(out), vals[idx])
+        ctx.scatter(dv, out, di, n, passes, wide=True)
+        sref = np.empty_like(vals)
+        sref[idx] = vals
+        assert np.array_equal(host(out), sref)
+    # unaligned int32 buffers use the scalar kernel
+    v32 = orc.gen_i32(n + 1, 11)
+    d32 = dev(torch, v32)
+    i32 = dev(torch, np.concatenate([[0], idx]))
+    o32 = empty(torch, n + 1, torch.int32)
+    ctx.gather(d32[1:], o32[1:], i32[1:], n, 3)
+    assert np.array_equal(host(o32)[1:], v32[1:][idx])
+
+
+def test_gather_out_of_range_index_is_skipped(torch_gpu, ctx, orc):
+    torch = torch_gpu
+    n = 4096
+    idx = orc.gen_perm(n, 1, 0)
+    idx[5] = n + 100
+    idx[9] = -3
+    vals = orc.gen_i32(n, 2)
+    out = torch.full((n,), -7, dtype=torch.int32, device="cuda")
+    ctx.gather(dev(torch, vals), out, dev(torch, idx), n, 1)
+    h = host(out)
+    ok = (idx >= 0) & (idx < n)
+    assert np.array_equal(h[ok], vals[idx[ok]]) and h[5] == -7 and h[9] == -7
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_gather_scatter_full_size_properties(torch_gpu, ctx, kind):
+    # 2^28: gather of iota returns the index (test_gather.cpp:64-70), and
+    # scatter undoes gather, for several pass counts
+    torch = torch_gpu
+    n = 1 << 28
+    idx = empty(torch, n, torch.int32)
+    ctx.gen_perm(idx, 31, kind)
+    iota = torch.arange(n, dtype=torch.int32, device="cuda")
+    out = empty(torch, n, torch.int32)
+    for passes in (1, 4):
+        ctx.gather(iota, out, idx, n, passes)
+        assert torch.equal(out, idx)
+        back = torch.zeros(n, dtype=torch.int32, device="cuda")
+        ctx.scatter(out, back, idx, n, passes)
+        assert torch.equal(back, iota)
+        del back
+    del idx, iota, out
+
+
+# ---------------------------------------------------------------- split
+SPLIT_SIZES = [0, 1, 63, 100, 4096 * 3 + 5, 100_003, 1 << 20]
+
+
+@pytest.mark.parametrize("strategy", [1, 2, 3])
+@pytest.mark.parametrize("B", [1, 2, 16, 256, 4096])
+@pytest.mark.parametrize("n", SPLIT_SIZES)
+def test_split_i32(torch_gpu, ctx, orc, strategy, B, n):
+    torch = torch_gpu
+    for dist in (0, 1):
+        keys, vals = orc.gen_split_keys(n, B, dist, 1000 + n + B)
+        ko, vo, start = orc.split(keys, B, vals)
+        dk, dvv = dev(torch, keys), dev(torch, vals)
+        kout = empty(torch, n, torch.int32)
+        vout = empty(torch, n, torch.int32)
+        st = torch.full((B,), -1, dtype=torch.int32, device="cuda")
+        ctx.split(dk, kout, B, dvv, vout, st, n, strategy)
+        assert np.array_equal(host(kout), ko), (strategy, B, n, dist)
+        assert np.array_equal(host(vout), vo), (strategy, B, n, dist)
+        assert np.array_equal(host(st), start), (strategy, B, n, dist)
+        # key-only
+        kout2 = empty(torch, n, torch.int32)
+        ctx.split(dk, kout2, B, n=n, strategy=strategy)
+        assert np.array_equal(host(kout2), ko)
+
+
+@pytest.mark.parametrize("strategy", [1, 2])
+@pytest.mark.parametrize("B", [2, 256, 4096])
+def test_split_i64(torch_gpu, ctx, orc, strategy, B):
+    torch = torch_gpu
+    n = 300_007
+    keys, vals = orc.gen_split_keys(n, B, 1, 5 + B, wide=True)
+    ko, vo, start = orc.split(keys, B, vals)
+    kout = empty(torch, n, torch.int64)
+    vout = empty(torch, n, torch.int32)
+    st = empty(torch, B, torch.int32)
+    ctx.split(dev(torch, keys), kout, B, dev(torch, vals), vout, st, n, strategy, wide=True)
+    assert np.array_equal(host(kout), ko) and np.array_equal(host(vout), vo) and np.array_equal(host(st), start)
+
+
+@pytest.mark.parametrize("B", [2, 16, 256, 4096])
+def test_split_full_size_properties(torch_gpu, ctx, B):
+    # 2^28 pairs, Zipf keys, vals = original index: bucket ids non-decreasing,
+    # stable (vals increasing inside each bucket), a permutation, keys follow
+    # their values, offsets equal the bucket counts
+    torch = torch_gpu
+    n = 1 << 28
+    keys = empty(torch, n, torch.int32)
+    vals = empty(torch, n, torch.int32)
+    ctx.gen_split_keys(keys, vals, B, 1, 4242)
+    kout = empty(torch, n, torch.int32)
+    vout = empty(torch, n, torch.int32)
+    st = empty(torch, B, torch.int32)
+    ctx.split(keys, kout, B, vals, vout, st, n, 1)
+    b = kout & (B - 1)
+    assert bool((b[1:] >= b[:-1]).all())
+    same = b[1:] == b[:-1]
+    assert bool((vout[1:][same] > vout[:-1][same]).all())
+    assert torch.equal(keys[vout.long()], kout)
+    cnt = torch.bincount((keys & (B - 1)).long(), minlength=B)
+    exp = torch.cumsum(cnt, 0) - cnt
+    assert torch.equal(st.long(), exp)
+    del keys, vals, kout, vout, b, same
