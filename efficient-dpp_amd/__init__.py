@@ -55,6 +55,19 @@ _SIGS = {
     "edpp_gen_split_keys_i32": [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_u64],
     "edpp_gen_split_keys_i64": [c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_u64],
 }
+
+
+class Tuning(ctypes.Structure):
+    """edpp_tuning of include/edpp_tune.h (0 / -1 fields mean the default)."""
+    _fields_ = [("gs_unroll", c_i32), ("gs_random_nt", c_i32), ("scan_vpt", c_i32), ("split_ipt", c_i32),
+                ("split_tile_mult", c_i32), ("reserved", c_i32 * 11)]
+
+
+TUNING_FIELDS = ("gs_unroll", "gs_random_nt", "scan_vpt", "split_ipt", "split_tile_mult")
+
+_SIGS["edpp_set_tuning"] = [c_vp, ctypes.POINTER(Tuning)]
+_SIGS["edpp_get_tuning"] = [c_vp, ctypes.POINTER(Tuning)]
+
 _RESTYPES = {"edpp_strerror": ctypes.c_char_p, "edpp_last_error": ctypes.c_char_p, "edpp_ctx_stream": c_vp}
 
 
@@ -100,7 +113,7 @@ class Context:
     """One library context: a HIP stream, a workspace and look-back state.
 
     ``stream`` is a raw hipStream_t address (for example
-    ``torch.cuda.current_stream().cuda_stream``) or None for a private stream.
+    ``torch.cuda.current_stream().cuda_stream``); None (0) is the default stream.
     """
 
     def __init__(self, device=0, stream=None):
@@ -141,6 +154,24 @@ class Context:
         ms = ctypes.c_double(0.0) if timed else None
         self._check(getattr(self._L, name)(self.h, *args, ctypes.byref(ms) if timed else None))
         return ms.value if timed else None
+
+    def set_tuning(self, **kw):
+        """Set schedule knobs (see include/edpp_tune.h); no arguments restores the defaults."""
+        if not kw:
+            self._check(self._L.edpp_set_tuning(self.h, None))
+            return
+        t = Tuning()
+        t.gs_random_nt = -1
+        for k, v in kw.items():
+            if k not in TUNING_FIELDS:
+                raise KeyError(k)
+            setattr(t, k, int(v))
+        self._check(self._L.edpp_set_tuning(self.h, ctypes.byref(t)))
+
+    def get_tuning(self):
+        t = Tuning()
+        self._check(self._L.edpp_get_tuning(self.h, ctypes.byref(t)))
+        return {k: getattr(t, k) for k in TUNING_FIELDS}
 
     def sync(self):
         self._check(self._L.edpp_ctx_sync(self.h))

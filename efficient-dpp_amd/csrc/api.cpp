@@ -172,12 +172,8 @@ int edpp_ctx_create(int device, void* stream, edpp_ctx** out) {
             c->xcds = xcc;
         // Infinity Cache of MI355X (not reported by the runtime)
         c->mall_bytes = (int64_t)256 << 20;
-        if (stream) {
-            c->stream = (hipStream_t)stream;
-        } else {
-            if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { rc = EDPP_EHIP; break; }
-            c->own_stream = true;
-        }
+        // NULL selects the device's default stream, as HIP and torch do
+        c->stream = (hipStream_t)stream;
         if (hipEventCreate(&c->ev_start) != hipSuccess || hipEventCreate(&c->ev_stop) != hipSuccess) {
             rc = EDPP_EHIP;
             break;
@@ -211,8 +207,36 @@ int edpp_ctx_destroy(edpp_ctx* c) {
     hipFree(c->d_err);
     if (c->ev_start) hipEventDestroy(c->ev_start);
     if (c->ev_stop) hipEventDestroy(c->ev_stop);
-    if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
     delete c;
+    return EDPP_OK;
+}
+
+int edpp_set_tuning(edpp_ctx* c, const edpp_tuning* t) {
+    EDPP_CHECK_CTX(c);
+    edpp_tuning d = edpp_ctx::default_tuning();
+    if (!t) {
+        c->tune = d;
+        return EDPP_OK;
+    }
+    edpp_tuning v = *t;
+    auto pick = [](int32_t x, int32_t dflt) { return x == 0 ? dflt : x; };
+    v.gs_unroll = pick(v.gs_unroll, d.gs_unroll);
+    v.gs_random_nt = v.gs_random_nt < 0 ? d.gs_random_nt : v.gs_random_nt;
+    v.scan_vpt = pick(v.scan_vpt, d.scan_vpt);
+    v.split_ipt = pick(v.split_ipt, d.split_ipt);
+    v.split_tile_mult = pick(v.split_tile_mult, d.split_tile_mult);
+    if ((v.gs_unroll != 1 && v.gs_unroll != 2 && v.gs_unroll != 4 && v.gs_unroll != 8) || v.gs_random_nt > 1 ||
+        (v.scan_vpt != 2 && v.scan_vpt != 4 && v.scan_vpt != 8) || (v.split_ipt != 8 && v.split_ipt != 16) ||
+        v.split_tile_mult < 16 || v.split_tile_mult > 1024)
+        return fail(c, EDPP_EINVAL, "tuning value out of range");
+    c->tune = v;
+    return EDPP_OK;
+}
+
+int edpp_get_tuning(edpp_ctx* c, edpp_tuning* t) {
+    EDPP_CHECK_CTX(c);
+    if (!t) return EDPP_EINVAL;
+    *t = c->tune;
     return EDPP_OK;
 }
 

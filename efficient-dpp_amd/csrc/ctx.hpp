@@ -4,12 +4,12 @@
 #include <stdint.h>
 #include <string>
 #include <vector>
-#include "../../include/edpp.h"
+#include "edpp.h"
+#include "edpp_tune.h"
 
 struct edpp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    bool own_stream = false;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     int cus = 0, xcds = 8;
     int64_t l2_bytes = 0, mall_bytes = 0;
@@ -29,7 +29,18 @@ struct edpp_ctx {
     uint32_t epoch = 1;                 // 30-bit, 0 never used
     uint32_t spin_limit = 1u << 22;     // look-back watchdog (polls before giving up, seconds)
 
+    edpp_tuning tune = default_tuning();
     std::string last_error;
+
+    static edpp_tuning default_tuning() {
+        edpp_tuning t = {};
+        t.gs_unroll = 4;
+        t.gs_random_nt = 0;
+        t.scan_vpt = 4;
+        t.split_ipt = 16;
+        t.split_tile_mult = 64;
+        return t;
+    }
 };
 
 namespace edpp {

@@ -46,7 +46,7 @@ __device__ __forceinline__ void store4(T* p, const T (&v)[4]) {
 
 // out[i] = in[idx[i]] for idx[i] in [from, from+span).  The host clamps
 // from+span to n, so an index outside [0,n) is skipped instead of read.
-template <class T, int U>
+template <class T, int U, bool RNT>
 __global__ __launch_bounds__(kGsBlock) void k_gather(const T* __restrict__ in, T* __restrict__ out,
                                                      const int32_t* __restrict__ idx, int64_t ngrp, uint32_t from,
                                                      uint32_t span) {
@@ -64,7 +64,8 @@ __global__ __launch_bounds__(kGsBlock) void k_gather(const T* __restrict__ in, T
     #pragma unroll
     for (int u = 0; u < U; ++u)
         #pragma unroll
-        for (int k = 0; k < 4; ++k) val[u][k] = (pos[u][k] - from < span) ? in[pos[u][k]] : T(0);
+        for (int k = 0; k < 4; ++k)
+            val[u][k] = (pos[u][k] - from < span) ? (RNT ? ld_stream(in + pos[u][k]) : in[pos[u][k]]) : T(0);
     #pragma unroll
     for (int u = 0; u < U; ++u) {
         int64_t g = g0 + (int64_t)u * kGsBlock;
@@ -84,7 +85,7 @@ __global__ __launch_bounds__(kGsBlock) void k_gather(const T* __restrict__ in, T
 }
 
 // out[idx[i]] = in[i] for idx[i] in [from, from+span)
-template <class T, int U>
+template <class T, int U, bool RNT>
 __global__ __launch_bounds__(kGsBlock) void k_scatter(const T* __restrict__ in, T* __restrict__ out,
                                                       const int32_t* __restrict__ idx, int64_t ngrp, uint32_t from,
                                                       uint32_t span) {
@@ -108,7 +109,10 @@ __global__ __launch_bounds__(kGsBlock) void k_scatter(const T* __restrict__ in, 
     for (int u = 0; u < U; ++u)
         #pragma unroll
         for (int k = 0; k < 4; ++k)
-            if (pos[u][k] - from < span) out[pos[u][k]] = val[u][k];
+            if (pos[u][k] - from < span) {
+                if (RNT) st_stream(val[u][k], out + pos[u][k]);
+                else out[pos[u][k]] = val[u][k];
+            }
 }
 
 // elements [lo, n): scalar tail and the path for unaligned pointers
@@ -125,28 +129,47 @@ __global__ void k_gs_scalar(const T* in, T* out, const int32_t* idx, int64_t lo,
     }
 }
 
+template <class T, int U, bool RNT>
+static void launch_gs_kernel(edpp_ctx* c, const T* in, T* out, const int32_t* idx, int64_t ngrp, uint32_t from,
+                             uint32_t span, bool scatter) {
+    const unsigned grid = (unsigned)ceil_div(ngrp, (int64_t)kGsBlock * U);
+    if (scatter)
+        hipLaunchKernelGGL((k_scatter<T, U, RNT>), dim3(grid), dim3(kGsBlock), 0, c->stream, in, out, idx, ngrp, from,
+                           span);
+    else
+        hipLaunchKernelGGL((k_gather<T, U, RNT>), dim3(grid), dim3(kGsBlock), 0, c->stream, in, out, idx, ngrp, from,
+                           span);
+}
+
+template <class T, bool RNT>
+static void dispatch_unroll(edpp_ctx* c, int u, const T* in, T* out, const int32_t* idx, int64_t ngrp, uint32_t from,
+                            uint32_t span, bool scatter) {
+    switch (u) {
+        case 1: launch_gs_kernel<T, 1, RNT>(c, in, out, idx, ngrp, from, span, scatter); break;
+        case 2: launch_gs_kernel<T, 2, RNT>(c, in, out, idx, ngrp, from, span, scatter); break;
+        case 8: launch_gs_kernel<T, 8, RNT>(c, in, out, idx, ngrp, from, span, scatter); break;
+        default: launch_gs_kernel<T, 4, RNT>(c, in, out, idx, ngrp, from, span, scatter); break;
+    }
+}
+
 template <class T>
 int launch_gather(edpp_ctx* c, const T* in, T* out, const int32_t* idx, int64_t n, int passes, bool scatter) {
     if (n == 0) return EDPP_OK;
     if (passes == 0) passes = default_passes(c, n, (int)sizeof(T), scatter);
     if (passes < 1 || passes > n) passes = passes < 1 ? 1 : (int)n;
-    constexpr int U = 4;
     const bool vec = aligned16(in) && aligned16(out) && aligned16(idx);
     const int64_t ngrp = vec ? n / 4 : 0;
     const int64_t len = ceil_div(n, passes);  // gatherImpl.cpp:37
-    const unsigned grid = (unsigned)ceil_div(ngrp, (int64_t)kGsBlock * U);
     for (int p = 0; p < passes; ++p) {
         const int64_t lo = (int64_t)p * len;
         if (lo >= n) break;
         const uint32_t from = (uint32_t)lo;
         const uint32_t span = (uint32_t)(lo + len <= n ? len : n - lo);
         if (ngrp) {
-            if (scatter)
-                hipLaunchKernelGGL((k_scatter<T, U>), dim3(grid), dim3(kGsBlock), 0, c->stream, in, out, idx, ngrp,
-                                   from, span);
+            if (c->tune.gs_random_nt > 0)
+                dispatch_unroll<T, true>(c, c->tune.gs_unroll, in, out, idx, ngrp, from, span, scatter);
             else
-                hipLaunchKernelGGL((k_gather<T, U>), dim3(grid), dim3(kGsBlock), 0, c->stream, in, out, idx, ngrp,
-                                   from, span);
+                dispatch_unroll<T, false>(c, c->tune.gs_unroll, in, out, idx, ngrp, from, span, scatter);
         }
         if (ngrp * 4 < n) {
             int64_t rest = n - ngrp * 4;

@@ -289,7 +289,10 @@ int launch_scan(edpp_ctx* c, const T* in, T* out, int64_t n, int scheme, T* d_to
     // tile of 4096 (u32) / 2048 (u64) elements for look-back; 8192 / 4096 for
     // the three-kernel schemes, whose partial scan is the serial bottleneck
     switch (scheme) {
-        case EDPP_SCAN_LOOKBACK: return run_lookback<T, 4>(c, in, out, n, d_total);
+        case EDPP_SCAN_LOOKBACK:
+            if (c->tune.scan_vpt == 2) return run_lookback<T, 2>(c, in, out, n, d_total);
+            if (c->tune.scan_vpt == 8) return run_lookback<T, 8>(c, in, out, n, d_total);
+            return run_lookback<T, 4>(c, in, out, n, d_total);
         case EDPP_SCAN_RTS: return run_rts_ssa<T, 8>(c, in, out, n, d_total, false);
         case EDPP_SCAN_SSA: return run_rts_ssa<T, 8>(c, in, out, n, d_total, true);
         default: return fail(c, EDPP_EINVAL, "unknown scan scheme %d", scheme);

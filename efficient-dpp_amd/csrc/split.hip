@@ -336,7 +336,7 @@ int run_wg(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t* vout
     constexpr int S = kSplitBlock * IPT;
     // tiles long enough that the [B][tiles] matrix stays small next to the data:
     // at least 64 elements per bucket per tile on average
-    int64_t subs = std::max<int64_t>(1, ((int64_t)64 * B + S - 1) / S);
+    int64_t subs = std::max<int64_t>(1, ((int64_t)c->tune.split_tile_mult * B + S - 1) / S);
     SplitGeom g;
     g.n = n;
     g.lg = lg;
@@ -456,7 +456,8 @@ int launch_split(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t
     switch (strategy) {
         case EDPP_SPLIT_WG:
             // keep the LDS footprint of the scatter under 160 KiB
-            if (wg_lds_bytes<16, K>(buckets) <= 160 * 1024) return run_wg<K, 16>(c, kin, kout, vin, vout, d_start, n, lg);
+            if (c->tune.split_ipt >= 16 && wg_lds_bytes<16, K>(buckets) <= 160 * 1024)
+                return run_wg<K, 16>(c, kin, kout, vin, vout, d_start, n, lg);
             return run_wg<K, 8>(c, kin, kout, vin, vout, d_start, n, lg);
         case EDPP_SPLIT_WI:
             return run_wi<K, 16>(c, kin, kout, vin, vout, d_start, n, lg);

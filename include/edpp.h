@@ -103,9 +103,9 @@ enum edpp_key_dist {
 typedef struct edpp_ctx edpp_ctx;
 
 /* ---- context ------------------------------------------------------------
- * One context per host thread.  `stream` may be NULL (the context creates a
- * default-priority stream of its own) or an existing hipStream_t to enqueue
- * on (for example the caller's framework stream).  The context owns a
+ * One context per host thread.  `stream` is the hipStream_t every call
+ * enqueues on; NULL means the device's default (null) stream, which is also
+ * what torch reports for its default stream.  The context owns a
  * device workspace that grows on demand (hipMalloc outside the timed
  * kernels, as the reference allocates its temporaries per call,
  * opencl/primitives/scanImpl.cpp:39, splitImpl.cpp:76).                    */

@@ -1,0 +1,37 @@
+/*
+ * edpp_tune.h — schedule knobs of the primitives, for the autotuner and the
+ * learned cost model.  Every knob only changes how the work is scheduled;
+ * results stay bit-exact for every setting.
+ *
+ * The reference tunes launch shapes by exhaustive search over local and
+ * grid sizes (opencl/test/test_split.cpp:273-358) and hand-picked scan
+ * settings (opencl/test/test_scan_global.cpp:87-94); these knobs are the
+ * MI355X counterparts (tile lengths, vectors per lane, cache hints).
+ */
+#ifndef EDPP_TUNE_H
+#define EDPP_TUNE_H
+
+#include <stdint.h>
+#include "edpp.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct edpp_tuning {
+    int32_t gs_unroll;        /* index vectors (4 elements each) per lane: 1, 2, 4, 8; 0 = default      */
+    int32_t gs_random_nt;     /* random side of gather/scatter with non-temporal hint: 0 / 1; -1 default */
+    int32_t scan_vpt;         /* 16-byte vectors per lane in a look-back tile: 2, 4, 8; 0 = default      */
+    int32_t split_ipt;        /* keys per lane per split sub-tile: 8 or 16; 0 = default                  */
+    int32_t split_tile_mult;  /* split tile >= mult * buckets elements: 16..1024; 0 = default (64)       */
+    int32_t reserved[11];
+} edpp_tuning;
+
+/* Applies to every later call on ctx; NULL restores the defaults. */
+int edpp_set_tuning(edpp_ctx* ctx, const edpp_tuning* t);
+int edpp_get_tuning(edpp_ctx* ctx, edpp_tuning* t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EDPP_TUNE_H */

@@ -102,7 +102,7 @@ def test_scan_in_place_and_unaligned(torch_gpu, ctx, orc, scheme):
     d = dev(torch, x)
     ctx.scan(d, d, n, scheme=scheme)  # in place, as split scans its histogram
     assert np.array_equal(host(d), ref)
-    d = dev(torch, np.concatenate([[0], x]))
+    d = dev(torch, np.concatenate([np.zeros(1, np.int32), x]))
     out = empty(torch, n + 1, torch.int32)
     ctx.scan(d[1:], out[1:], n, scheme=scheme)  # 4-byte offset: not 16-byte aligned
     assert np.array_equal(host(out)[1:], ref)
@@ -195,7 +195,7 @@ def test_gather_scatter_i64_and_unaligned(torch_gpu, ctx, orc, n):
     # unaligned int32 buffers use the scalar kernel
     v32 = orc.gen_i32(n + 1, 11)
     d32 = dev(torch, v32)
-    i32 = dev(torch, np.concatenate([[0], idx]))
+    i32 = dev(torch, np.concatenate([np.zeros(1, np.int32), idx]))
     o32 = empty(torch, n + 1, torch.int32)
     ctx.gather(d32[1:], o32[1:], i32[1:], n, 3)
     assert np.array_equal(host(o32)[1:], v32[1:][idx])
