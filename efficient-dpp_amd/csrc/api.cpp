@@ -225,9 +225,11 @@ int edpp_set_tuning(edpp_ctx* c, const edpp_tuning* t) {
     v.scan_vpt = pick(v.scan_vpt, d.scan_vpt);
     v.split_ipt = pick(v.split_ipt, d.split_ipt);
     v.split_tile_mult = pick(v.split_tile_mult, d.split_tile_mult);
+    v.scan_block = pick(v.scan_block, d.scan_block);
     if ((v.gs_unroll != 1 && v.gs_unroll != 2 && v.gs_unroll != 4 && v.gs_unroll != 8) || v.gs_random_nt > 1 ||
-        (v.scan_vpt != 2 && v.scan_vpt != 4 && v.scan_vpt != 8) || (v.split_ipt != 8 && v.split_ipt != 16) ||
-        v.split_tile_mult < 16 || v.split_tile_mult > 1024)
+        (v.scan_vpt != 2 && v.scan_vpt != 4 && v.scan_vpt != 8 && v.scan_vpt != 16) || (v.split_ipt != 8 && v.split_ipt != 16) ||
+        v.split_tile_mult < 16 || v.split_tile_mult > 1024 ||
+        (v.scan_block != 256 && v.scan_block != 512 && v.scan_block != 1024))
         return fail(c, EDPP_EINVAL, "tuning value out of range");
     c->tune = v;
     return EDPP_OK;

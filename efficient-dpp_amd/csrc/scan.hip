@@ -224,14 +224,14 @@ template <> LookbackState64 make_state<uint64_t>(edpp_ctx* c, uint32_t epoch) {
 
 constexpr int kBlock = 256;
 
-template <class T, int VPT>
+template <class T, int BLOCK, int VPT>
 int run_lookback(edpp_ctx* c, const T* in, T* out, int64_t n, T* d_total) {
-    typedef TileScan<T, kBlock, VPT> TS;
+    typedef TileScan<T, BLOCK, VPT> TS;
     int64_t tiles = ceil_div(n, TS::TILE);
     uint32_t epoch;
     int rc = scan_state_acquire(c, tiles, &epoch);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_scan_lookback<T, kBlock, VPT>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, in, out,
+    hipLaunchKernelGGL((k_scan_lookback<T, BLOCK, VPT>), dim3((unsigned)tiles), dim3(BLOCK), 0, c->stream, in, out,
                        n, make_state<T>(c, epoch), c->tickets, d_total, c->spin_limit, c->d_err, tiles);
     EDPP_HIP(c, hipGetLastError());
     return EDPP_OK;
@@ -289,10 +289,22 @@ int launch_scan(edpp_ctx* c, const T* in, T* out, int64_t n, int scheme, T* d_to
     // tile of 4096 (u32) / 2048 (u64) elements for look-back; 8192 / 4096 for
     // the three-kernel schemes, whose partial scan is the serial bottleneck
     switch (scheme) {
-        case EDPP_SCAN_LOOKBACK:
-            if (c->tune.scan_vpt == 2) return run_lookback<T, 2>(c, in, out, n, d_total);
-            if (c->tune.scan_vpt == 8) return run_lookback<T, 8>(c, in, out, n, d_total);
-            return run_lookback<T, 4>(c, in, out, n, d_total);
+        case EDPP_SCAN_LOOKBACK: {
+            const int v = c->tune.scan_vpt, b = c->tune.scan_block;
+            if (b == 512) {
+                if (v <= 4) return run_lookback<T, 512, 4>(c, in, out, n, d_total);
+                if (v == 8) return run_lookback<T, 512, 8>(c, in, out, n, d_total);
+                return run_lookback<T, 512, 16>(c, in, out, n, d_total);
+            }
+            if (b == 1024) {
+                if (v <= 4) return run_lookback<T, 1024, 4>(c, in, out, n, d_total);
+                return run_lookback<T, 1024, 8>(c, in, out, n, d_total);
+            }
+            if (v == 2) return run_lookback<T, 256, 2>(c, in, out, n, d_total);
+            if (v == 4) return run_lookback<T, 256, 4>(c, in, out, n, d_total);
+            if (v == 8) return run_lookback<T, 256, 8>(c, in, out, n, d_total);
+            return run_lookback<T, 256, 16>(c, in, out, n, d_total);
+        }
         case EDPP_SCAN_RTS: return run_rts_ssa<T, 8>(c, in, out, n, d_total, false);
         case EDPP_SCAN_SSA: return run_rts_ssa<T, 8>(c, in, out, n, d_total, true);
         default: return fail(c, EDPP_EINVAL, "unknown scan scheme %d", scheme);

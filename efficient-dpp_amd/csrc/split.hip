@@ -63,6 +63,7 @@ __host__ __device__ __forceinline__ int kbuf_word_offset(int B) {
 struct SplitGeom {
     int64_t n;
     int lg;          // log2 B
+    int shift;       // bucket = (key >> shift) & (B-1)
     int64_t tile;    // elements per tile
     int64_t tiles;
     int xcds;
@@ -70,7 +71,7 @@ struct SplitGeom {
 
 // per-wave counts of one sub-tile into whist[w][b] (zeroed by the caller)
 template <class K, int IPT>
-__device__ __forceinline__ void rank_subtile(const K* __restrict__ kin, int64_t sbase, int64_t n, int lg,
+__device__ __forceinline__ void rank_subtile(const K* __restrict__ kin, int64_t sbase, int64_t n, int lg, int shift,
                                              uint32_t* whist_w, uint32_t (&bk)[IPT], uint32_t (&rk)[IPT],
                                              K (&key)[IPT]) {
     const int lane = lane_id();
@@ -82,7 +83,7 @@ __device__ __forceinline__ void rank_subtile(const K* __restrict__ kin, int64_t 
         int64_t e = wbase + (int64_t)j * kWave + lane;
         bool ok = e < n;
         key[j] = ok ? kin[e] : K(0);
-        bk[j] = ok ? ((uint32_t)key[j] & mask) : 0xFFFFFFFFu;
+        bk[j] = ok ? (((uint32_t)key[j] >> shift) & mask) : 0xFFFFFFFFu;
     }
     #pragma unroll
     for (int j = 0; j < IPT; ++j) {
@@ -138,7 +139,7 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_hist(const K* __restrict_
     for (int64_t s = tbase; s < tend; s += S) {
         uint32_t bk[IPT], rk[IPT];
         K key[IPT];
-        rank_subtile<K, IPT>(kin, s, tend, g.lg, whist + w * B, bk, rk, key);
+        rank_subtile<K, IPT>(kin, s, tend, g.lg, g.shift, whist + w * B, bk, rk, key);
     }
     __syncthreads();
     for (int b = threadIdx.x; b < B; b += kSplitBlock) {
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_scatter(const K* __restri
         __syncthreads();
         uint32_t bk[IPT], rk[IPT];
         K key[IPT];
-        rank_subtile<K, IPT>(kin, s, tend, g.lg, whist + w * B, bk, rk, key);
+        rank_subtile<K, IPT>(kin, s, tend, g.lg, g.shift, whist + w * B, bk, rk, key);
         int32_t val[IPT];
         if (KV) {
             const int64_t wbase = s + (int64_t)w * kWave * IPT;
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_scatter(const K* __restri
         const int cnt = (int)std::min<int64_t>((int64_t)S, tend - s);
         for (int i = threadIdx.x; i < cnt; i += kSplitBlock) {
             K k = kbuf[i];
-            uint32_t b = (uint32_t)k & mask;
+            uint32_t b = ((uint32_t)k >> g.shift) & mask;
             uint32_t pos = run[b] + ((uint32_t)i - off[b]);
             if (pos < (uint64_t)g.n) {  // always true for a consistent histogram scan
                 kout[pos] = k;
@@ -236,10 +237,15 @@ __global__ void k_split_start(const int32_t* scanned, int64_t tiles, int B, int3
 }
 
 // ------------------------------------------------------------- WI strategy
-// One column of the histogram per wave chunk of 64*IPT elements.
+// One histogram column per wave: the wave walks `colen` consecutive elements
+// (a multiple of 64*IPT) in rounds, keeping per-bucket counts in its own LDS
+// row, so no cross-wave work is needed.  The scatter starts each row at the
+// scanned column offsets, which turns the running counts into absolute
+// output positions.
 template <class K, int IPT>
 __global__ __launch_bounds__(kSplitBlock) void k_split_wave_hist(const K* __restrict__ kin, int64_t n, int lg,
-                                                                 int64_t cols, int32_t* __restrict__ his) {
+                                                                 int shift, int64_t cols, int64_t colen,
+                                                                 int32_t* __restrict__ his) {
     extern __shared__ uint32_t lds[];
     const int B = 1 << lg;
     const int w = threadIdx.x / kWave;
@@ -247,22 +253,23 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_wave_hist(const K* __rest
     const int64_t col = (int64_t)blockIdx.x * kSplitWaves + w;
     for (int b = lane_id(); b < B; b += kWave) h[b] = 0;
     __builtin_amdgcn_wave_barrier();
-    if (col < cols) {
+    if (col >= cols) return;
+    const int64_t c0 = col * colen, c1 = std::min<int64_t>(c0 + colen, n);
+    for (int64_t s = c0; s < c1; s += (int64_t)kWave * IPT) {
         uint32_t bk[IPT], rk[IPT];
         K key[IPT];
-        // rank_subtile addresses the wave's chunk as sbase + w*64*IPT
-        const int64_t sbase = col * kWave * IPT - (int64_t)w * kWave * IPT;
-        rank_subtile<K, IPT>(kin, sbase, n, lg, h, bk, rk, key);
-        __builtin_amdgcn_wave_barrier();
-        for (int b = lane_id(); b < B; b += kWave) his[(int64_t)b * cols + col] = (int32_t)h[b];
+        // rank_subtile addresses the calling wave's chunk at sbase + w*64*IPT
+        rank_subtile<K, IPT>(kin, s - (int64_t)w * kWave * IPT, c1, lg, shift, h, bk, rk, key);
     }
+    __builtin_amdgcn_wave_barrier();
+    for (int b = lane_id(); b < B; b += kWave) his[(int64_t)b * cols + col] = (int32_t)h[b];
 }
 
 template <class K, int IPT, bool KV>
 __global__ __launch_bounds__(kSplitBlock) void k_split_wave_scatter(const K* __restrict__ kin, K* __restrict__ kout,
                                                                     const int32_t* __restrict__ vin,
                                                                     int32_t* __restrict__ vout, int64_t n, int lg,
-                                                                    int64_t cols,
+                                                                    int shift, int64_t cols, int64_t colen,
                                                                     const int32_t* __restrict__ scanned) {
     extern __shared__ uint32_t lds[];
     const int B = 1 << lg;
@@ -270,22 +277,19 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_wave_scatter(const K* __r
     const int lane = lane_id();
     uint32_t* h = lds + w * B;
     const int64_t col = (int64_t)blockIdx.x * kSplitWaves + w;
-    for (int b = lane; b < B; b += kWave) h[b] = 0;
-    __builtin_amdgcn_wave_barrier();
     if (col >= cols) return;
-    uint32_t bk[IPT], rk[IPT];
-    K key[IPT];
-    const int64_t sbase = col * kWave * IPT - (int64_t)w * kWave * IPT;
-    rank_subtile<K, IPT>(kin, sbase, n, lg, h, bk, rk, key);
-    const int64_t cbase = col * kWave * IPT;
-    #pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-        if (bk[j] != 0xFFFFFFFFu) {
-            int64_t e = cbase + (int64_t)j * kWave + lane;
-            uint32_t pos = (uint32_t)scanned[(int64_t)bk[j] * cols + col] + rk[j];
-            if (pos < (uint64_t)n) {
-                kout[pos] = key[j];
-                if (KV) vout[pos] = vin[e];
+    for (int b = lane; b < B; b += kWave) h[b] = (uint32_t)scanned[(int64_t)b * cols + col];
+    __builtin_amdgcn_wave_barrier();
+    const int64_t c0 = col * colen, c1 = std::min<int64_t>(c0 + colen, n);
+    for (int64_t s = c0; s < c1; s += (int64_t)kWave * IPT) {
+        uint32_t bk[IPT], rk[IPT];
+        K key[IPT];
+        rank_subtile<K, IPT>(kin, s - (int64_t)w * kWave * IPT, c1, lg, shift, h, bk, rk, key);
+        #pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            if (bk[j] != 0xFFFFFFFFu && rk[j] < (uint64_t)n) {  // rk is the output position
+                kout[rk[j]] = key[j];
+                if (KV) vout[rk[j]] = vin[s + (int64_t)j * kWave + lane];
             }
         }
     }
@@ -294,24 +298,25 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_wave_scatter(const K* __r
 // ------------------------------------------------------------- SINGLE strategy
 // Lane g owns [g*len, (g+1)*len) and keeps its histogram column his[b][g].
 template <class K>
-__global__ void k_split_single_hist(const K* kin, int64_t n, int lg, int64_t len, int64_t lanes, int32_t* his) {
+__global__ void k_split_single_hist(const K* kin, int64_t n, int lg, int shift, int64_t len, int64_t lanes,
+                                    int32_t* his) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= lanes) return;
     const int B = 1 << lg;
     for (int b = 0; b < B; ++b) his[(int64_t)b * lanes + g] = 0;
     const int64_t e0 = g * len, e1 = std::min<int64_t>(e0 + len, n);
-    for (int64_t e = e0; e < e1; ++e) his[(int64_t)((uint32_t)kin[e] & (B - 1)) * lanes + g] += 1;
+    for (int64_t e = e0; e < e1; ++e) his[(int64_t)(((uint32_t)kin[e] >> shift) & (B - 1)) * lanes + g] += 1;
 }
 template <class K, bool KV>
 __global__ void k_split_single_scatter(const K* kin, K* kout, const int32_t* vin, int32_t* vout, int64_t n, int lg,
-                                       int64_t len, int64_t lanes, int32_t* scanned) {
+                                       int shift, int64_t len, int64_t lanes, int32_t* scanned) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= lanes) return;
     const int B = 1 << lg;
     const int64_t e0 = g * len, e1 = std::min<int64_t>(e0 + len, n);
     for (int64_t e = e0; e < e1; ++e) {
         K k = kin[e];
-        int32_t* slot = scanned + (int64_t)((uint32_t)k & (B - 1)) * lanes + g;
+        int32_t* slot = scanned + (int64_t)(((uint32_t)k >> shift) & (B - 1)) * lanes + g;
         int32_t p = *slot;
         *slot = p + 1;
         if ((uint32_t)p < (uint64_t)n) {
@@ -319,6 +324,26 @@ __global__ void k_split_single_scatter(const K* kin, K* kout, const int32_t* vin
             if (KV) vout[p] = vin[e];
         }
     }
+}
+
+// counts[b] += number of keys with ((key >> 0) & (B-1)) == b (order-free)
+template <class K>
+__global__ __launch_bounds__(256) void k_bucket_count(const K* __restrict__ kin, int64_t n, int lg,
+                                                      uint32_t* __restrict__ counts) {
+    extern __shared__ uint32_t h[];
+    const int B = 1 << lg;
+    for (int b = threadIdx.x; b < B; b += blockDim.x) h[b] = 0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&h[(uint32_t)kin[i] & (uint32_t)(B - 1)], 1u);
+    __syncthreads();
+    for (int b = threadIdx.x; b < B; b += blockDim.x)
+        if (h[b]) atomicAdd(&counts[b], h[b]);
+}
+
+__global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
 // ------------------------------------------------------------- host side
@@ -329,26 +354,57 @@ size_t wg_lds_bytes(int B) {
     return (size_t)kbuf_word_offset(B) * 4 + (size_t)kSplitBlock * IPT * (sizeof(K) + 4);
 }
 
+template <class K>
+struct SplitCall {
+    const K* kin;
+    K* kout;
+    const int32_t* vin;
+    int32_t* vout;
+    int32_t* d_start;  // start offsets of the (lg, shift) buckets, may be null
+    int64_t n;
+    int lg, shift;
+};
+
+int64_t wg_tile(edpp_ctx* c, int lg, int S) {
+    int64_t subs = std::max<int64_t>(1, ((int64_t)c->tune.split_tile_mult * (1 << lg) + S - 1) / S);
+    return subs * S;
+}
+int64_t wi_colen(edpp_ctx* c, int lg, int ipt) {
+    const int64_t R = (int64_t)kWave * ipt;
+    return std::max<int64_t>(R, ceil_div((int64_t)c->tune.split_tile_mult * (1 << lg), R) * R);
+}
+int64_t single_lanes(edpp_ctx* c, int64_t n, int lg) {
+    int64_t lanes = std::min<int64_t>((int64_t)c->cus * 64, std::max<int64_t>(1, n / 64));
+    while (lanes > 1 && ((int64_t)1 << lg) * lanes >= (1ll << 28)) lanes /= 2;
+    return lanes;
+}
+
+template <class K>
+int wg_ipt(edpp_ctx* c, int lg) {
+    return (c->tune.split_ipt >= 16 && wg_lds_bytes<16, K>(1 << lg) <= 160 * 1024) ? 16 : 8;
+}
+
+// histogram entries one pass needs
+template <class K>
+int64_t his_len(edpp_ctx* c, int strategy, int64_t n, int lg) {
+    const int64_t B = (int64_t)1 << lg;
+    if (strategy == EDPP_SPLIT_WG) return B * ceil_div(n, wg_tile(c, lg, kSplitBlock * wg_ipt<K>(c, lg)));
+    if (strategy == EDPP_SPLIT_WI) return B * ceil_div(n, wi_colen(c, lg, 16));
+    return B * single_lanes(c, n, lg);
+}
+
 template <class K, int IPT>
-int run_wg(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t* vout, int32_t* d_start, int64_t n,
-           int lg) {
-    const int B = 1 << lg;
+int run_wg(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
+    const int B = 1 << a.lg;
     constexpr int S = kSplitBlock * IPT;
-    // tiles long enough that the [B][tiles] matrix stays small next to the data:
-    // at least 64 elements per bucket per tile on average
-    int64_t subs = std::max<int64_t>(1, ((int64_t)c->tune.split_tile_mult * B + S - 1) / S);
     SplitGeom g;
-    g.n = n;
-    g.lg = lg;
-    g.tile = subs * S;
-    g.tiles = ceil_div(n, g.tile);
+    g.n = a.n;
+    g.lg = a.lg;
+    g.shift = a.shift;
+    g.tile = wg_tile(c, a.lg, S);
+    g.tiles = ceil_div(a.n, g.tile);
     g.xcds = c->xcds;
     const int64_t hlen = (int64_t)B * g.tiles;
-    if (hlen >= (1ll << 31)) return fail(c, EDPP_EUNSUP, "histogram of %lld entries too large", (long long)hlen);
-    void* w;
-    int rc = ws_reserve(c, (size_t)hlen * 4 + 256, &w);
-    if (rc) return rc;
-    int32_t* his = static_cast<int32_t*>(w);
     size_t lh = (size_t)(kSplitWaves + 1) * B * 4;
     size_t ls = wg_lds_bytes<IPT, K>(B);
     EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_hist<K, IPT>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -357,34 +413,30 @@ int run_wg(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t* vout
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls));
     EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_scatter<K, IPT, false>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls));
-    hipLaunchKernelGGL((k_split_hist<K, IPT>), dim3((unsigned)g.tiles), dim3(kSplitBlock), lh, c->stream, kin, g, his);
+    hipLaunchKernelGGL((k_split_hist<K, IPT>), dim3((unsigned)g.tiles), dim3(kSplitBlock), lh, c->stream, a.kin, g,
+                       his);
     EDPP_HIP(c, hipGetLastError());
-    rc = launch_scan<uint32_t>(c, (const uint32_t*)his, (uint32_t*)his, hlen, EDPP_SCAN_LOOKBACK, nullptr);
+    int rc = launch_scan<uint32_t>(c, (const uint32_t*)his, (uint32_t*)his, hlen, EDPP_SCAN_LOOKBACK, nullptr);
     if (rc) return rc;
-    if (d_start)
+    if (a.d_start)
         hipLaunchKernelGGL(k_split_start, dim3((unsigned)ceil_div(B, 256)), dim3(256), 0, c->stream,
-                           (const int32_t*)his, g.tiles, B, d_start);
-    if (vin)
+                           (const int32_t*)his, g.tiles, B, a.d_start);
+    if (a.vin)
         hipLaunchKernelGGL((k_split_scatter<K, IPT, true>), dim3((unsigned)g.tiles), dim3(kSplitBlock), ls, c->stream,
-                           kin, kout, vin, vout, g, (const int32_t*)his);
+                           a.kin, a.kout, a.vin, a.vout, g, (const int32_t*)his);
     else
         hipLaunchKernelGGL((k_split_scatter<K, IPT, false>), dim3((unsigned)g.tiles), dim3(kSplitBlock), ls,
-                           c->stream, kin, kout, vin, vout, g, (const int32_t*)his);
+                           c->stream, a.kin, a.kout, a.vin, a.vout, g, (const int32_t*)his);
     EDPP_HIP(c, hipGetLastError());
     return EDPP_OK;
 }
 
 template <class K, int IPT>
-int run_wi(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t* vout, int32_t* d_start, int64_t n,
-           int lg) {
-    const int B = 1 << lg;
-    const int64_t cols = ceil_div(n, (int64_t)kWave * IPT);
+int run_wi(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
+    const int B = 1 << a.lg;
+    const int64_t colen = wi_colen(c, a.lg, IPT);
+    const int64_t cols = ceil_div(a.n, colen);
     const int64_t hlen = (int64_t)B * cols;
-    if (hlen >= (1ll << 31)) return fail(c, EDPP_EUNSUP, "histogram of %lld entries too large", (long long)hlen);
-    void* w;
-    int rc = ws_reserve(c, (size_t)hlen * 4 + 256, &w);
-    if (rc) return rc;
-    int32_t* his = static_cast<int32_t*>(w);
     const unsigned grid = (unsigned)ceil_div(cols, kSplitWaves);
     const size_t lds = (size_t)kSplitWaves * B * 4;
     EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_wave_hist<K, IPT>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -393,55 +445,66 @@ int run_wi(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t* vout
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_wave_scatter<K, IPT, false>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_split_wave_hist<K, IPT>), dim3(grid), dim3(kSplitBlock), lds, c->stream, kin, n, lg, cols,
-                       his);
+    hipLaunchKernelGGL((k_split_wave_hist<K, IPT>), dim3(grid), dim3(kSplitBlock), lds, c->stream, a.kin, a.n, a.lg,
+                       a.shift, cols, colen, his);
     EDPP_HIP(c, hipGetLastError());
-    rc = launch_scan<uint32_t>(c, (const uint32_t*)his, (uint32_t*)his, hlen, EDPP_SCAN_LOOKBACK, nullptr);
+    int rc = launch_scan<uint32_t>(c, (const uint32_t*)his, (uint32_t*)his, hlen, EDPP_SCAN_LOOKBACK, nullptr);
     if (rc) return rc;
-    if (d_start)
+    if (a.d_start)
         hipLaunchKernelGGL(k_split_start, dim3((unsigned)ceil_div(B, 256)), dim3(256), 0, c->stream,
-                           (const int32_t*)his, cols, B, d_start);
-    if (vin)
-        hipLaunchKernelGGL((k_split_wave_scatter<K, IPT, true>), dim3(grid), dim3(kSplitBlock), lds, c->stream, kin,
-                           kout, vin, vout, n, lg, cols, (const int32_t*)his);
+                           (const int32_t*)his, cols, B, a.d_start);
+    if (a.vin)
+        hipLaunchKernelGGL((k_split_wave_scatter<K, IPT, true>), dim3(grid), dim3(kSplitBlock), lds, c->stream, a.kin,
+                           a.kout, a.vin, a.vout, a.n, a.lg, a.shift, cols, colen, (const int32_t*)his);
     else
-        hipLaunchKernelGGL((k_split_wave_scatter<K, IPT, false>), dim3(grid), dim3(kSplitBlock), lds, c->stream, kin,
-                           kout, vin, vout, n, lg, cols, (const int32_t*)his);
+        hipLaunchKernelGGL((k_split_wave_scatter<K, IPT, false>), dim3(grid), dim3(kSplitBlock), lds, c->stream,
+                           a.kin, a.kout, a.vin, a.vout, a.n, a.lg, a.shift, cols, colen, (const int32_t*)his);
     EDPP_HIP(c, hipGetLastError());
     return EDPP_OK;
 }
 
 template <class K>
-int run_single(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t* vout, int32_t* d_start, int64_t n,
-               int lg) {
-    const int B = 1 << lg;
-    int64_t lanes = std::min<int64_t>((int64_t)c->cus * 64, std::max<int64_t>(1, n / 64));
-    while (lanes > 1 && (int64_t)B * lanes >= (1ll << 28)) lanes /= 2;
-    const int64_t len = ceil_div(n, lanes);
+int run_single(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
+    const int B = 1 << a.lg;
+    const int64_t lanes = single_lanes(c, a.n, a.lg);
+    const int64_t len = ceil_div(a.n, lanes);
     const int64_t hlen = (int64_t)B * lanes;
-    void* w;
-    int rc = ws_reserve(c, (size_t)hlen * 4 + 256, &w);
-    if (rc) return rc;
-    int32_t* his = static_cast<int32_t*>(w);
     const unsigned grid = (unsigned)ceil_div(lanes, 64);
-    hipLaunchKernelGGL((k_split_single_hist<K>), dim3(grid), dim3(64), 0, c->stream, kin, n, lg, len, lanes, his);
+    hipLaunchKernelGGL((k_split_single_hist<K>), dim3(grid), dim3(64), 0, c->stream, a.kin, a.n, a.lg, a.shift, len,
+                       lanes, his);
     EDPP_HIP(c, hipGetLastError());
-    rc = launch_scan<uint32_t>(c, (const uint32_t*)his, (uint32_t*)his, hlen, EDPP_SCAN_LOOKBACK, nullptr);
+    int rc = launch_scan<uint32_t>(c, (const uint32_t*)his, (uint32_t*)his, hlen, EDPP_SCAN_LOOKBACK, nullptr);
     if (rc) return rc;
-    if (d_start)
+    if (a.d_start)
         hipLaunchKernelGGL(k_split_start, dim3((unsigned)ceil_div(B, 256)), dim3(256), 0, c->stream,
-                           (const int32_t*)his, lanes, B, d_start);
-    if (vin)
-        hipLaunchKernelGGL((k_split_single_scatter<K, true>), dim3(grid), dim3(64), 0, c->stream, kin, kout, vin, vout,
-                           n, lg, len, lanes, his);
+                           (const int32_t*)his, lanes, B, a.d_start);
+    if (a.vin)
+        hipLaunchKernelGGL((k_split_single_scatter<K, true>), dim3(grid), dim3(64), 0, c->stream, a.kin, a.kout, a.vin,
+                           a.vout, a.n, a.lg, a.shift, len, lanes, his);
     else
-        hipLaunchKernelGGL((k_split_single_scatter<K, false>), dim3(grid), dim3(64), 0, c->stream, kin, kout, vin,
-                           vout, n, lg, len, lanes, his);
+        hipLaunchKernelGGL((k_split_single_scatter<K, false>), dim3(grid), dim3(64), 0, c->stream, a.kin, a.kout,
+                           a.vin, a.vout, a.n, a.lg, a.shift, len, lanes, his);
     EDPP_HIP(c, hipGetLastError());
     return EDPP_OK;
 }
 
+template <class K>
+int run_pass(edpp_ctx* c, int strategy, const SplitCall<K>& a, int32_t* his) {
+    switch (strategy) {
+        case EDPP_SPLIT_WG:
+            if (wg_ipt<K>(c, a.lg) == 16) return run_wg<K, 16>(c, a, his);
+            return run_wg<K, 8>(c, a, his);
+        case EDPP_SPLIT_WI: return run_wi<K, 16>(c, a, his);
+        default: return run_single<K>(c, a, his);
+    }
+}
+
 }  // namespace
+
+// Bucket counts above kMaxSplitBitsPerPass bits are split as two stable LSD
+// passes (low bits first), whose runs per bucket stay long enough for
+// coalesced stores; the composition is the stable split by the full bucket.
+constexpr int kMaxSplitBitsPerPass = 8;
 
 template <class K>
 int launch_split(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t* vout, int32_t* d_start, int64_t n,
@@ -453,19 +516,45 @@ int launch_split(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t
         return EDPP_OK;
     }
     if (strategy == EDPP_SPLIT_AUTO) strategy = default_split_strategy(c, n, buckets);
-    switch (strategy) {
-        case EDPP_SPLIT_WG:
-            // keep the LDS footprint of the scatter under 160 KiB
-            if (c->tune.split_ipt >= 16 && wg_lds_bytes<16, K>(buckets) <= 160 * 1024)
-                return run_wg<K, 16>(c, kin, kout, vin, vout, d_start, n, lg);
-            return run_wg<K, 8>(c, kin, kout, vin, vout, d_start, n, lg);
-        case EDPP_SPLIT_WI:
-            return run_wi<K, 16>(c, kin, kout, vin, vout, d_start, n, lg);
-        case EDPP_SPLIT_SINGLE:
-            return run_single<K>(c, kin, kout, vin, vout, d_start, n, lg);
-        default:
-            return fail(c, EDPP_EINVAL, "unknown split strategy %d", strategy);
+    if (strategy < EDPP_SPLIT_WG || strategy > EDPP_SPLIT_SINGLE)
+        return fail(c, EDPP_EINVAL, "unknown split strategy %d", strategy);
+    const bool two = strategy != EDPP_SPLIT_SINGLE && lg > kMaxSplitBitsPerPass;
+    const int lg1 = two ? lg / 2 : lg, lg2 = lg - lg1;
+    int64_t hlen = his_len<K>(c, strategy, n, lg1);
+    if (two) hlen = std::max(hlen, his_len<K>(c, strategy, n, lg2));
+    if (hlen >= (1ll << 31)) return fail(c, EDPP_EUNSUP, "histogram of %lld entries too large", (long long)hlen);
+    const size_t hbytes = ((size_t)hlen * 4 + 255) & ~(size_t)255;
+    const size_t kbytes = two ? (((size_t)n * sizeof(K) + 255) & ~(size_t)255) : 0;
+    const size_t vbytes = (two && vin) ? (((size_t)n * 4 + 255) & ~(size_t)255) : 0;
+    const size_t cbytes = (two && d_start) ? 256 * 64 : 0;  // 4096 counters
+    void* w;
+    int rc = ws_reserve(c, hbytes + kbytes + vbytes + cbytes, &w);
+    if (rc) return rc;
+    char* base = static_cast<char*>(w);
+    int32_t* his = reinterpret_cast<int32_t*>(base);
+    if (!two) {
+        SplitCall<K> a = {kin, kout, vin, vout, d_start, n, lg, 0};
+        return run_pass<K>(c, strategy, a, his);
     }
+    K* tk = reinterpret_cast<K*>(base + hbytes);
+    int32_t* tv = vin ? reinterpret_cast<int32_t*>(base + hbytes + kbytes) : nullptr;
+    SplitCall<K> a1 = {kin, tk, vin, tv, nullptr, n, lg1, 0};
+    rc = run_pass<K>(c, strategy, a1, his);
+    if (rc) return rc;
+    SplitCall<K> a2 = {tk, kout, tv, vout, nullptr, n, lg2, lg1};
+    rc = run_pass<K>(c, strategy, a2, his);
+    if (rc) return rc;
+    if (d_start) {
+        uint32_t* cnt = reinterpret_cast<uint32_t*>(base + hbytes + kbytes + vbytes);
+        hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)ceil_div(buckets, 256)), dim3(256), 0, c->stream, cnt,
+                           (int64_t)buckets, 0u);
+        const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(n, 256), (int64_t)c->cus * 4);
+        hipLaunchKernelGGL((k_bucket_count<K>), dim3(grid), dim3(256), (size_t)buckets * 4, c->stream, kin, n, lg, cnt);
+        EDPP_HIP(c, hipGetLastError());
+        rc = launch_scan<uint32_t>(c, cnt, (uint32_t*)d_start, buckets, EDPP_SCAN_LOOKBACK, nullptr);
+        if (rc) return rc;
+    }
+    return EDPP_OK;
 }
 
 template int launch_split<uint32_t>(edpp_ctx*, const uint32_t*, uint32_t*, const int32_t*, int32_t*, int32_t*, int64_t,

@@ -159,44 +159,56 @@ template <class T> struct LookbackFor;
 template <> struct LookbackFor<uint32_t> { typedef LookbackState32 type; };
 template <> struct LookbackFor<uint64_t> { typedef LookbackState64 type; };
 
-// Decoupled look-back, executed by one whole wave for tile `tile` > 0.  Lane l
-// inspects predecessor (cursor - l); the window advances by 64 until a tile
-// with an inclusive prefix is found.  Waiting only on smaller tickets, which
-// belong to workgroups that are already running, guarantees progress.
-// Returns the exclusive prefix (valid in every lane).  On watchdog expiry
-// sets *err and returns what it has (the caller's result is then invalid and
-// the host reports EDPP_ETIMEOUT).
+// Decoupled look-back, executed by one whole wave for tile `tile` > 0.  One
+// round trip reads kLookbackWindows windows of 64 predecessors (lane l of
+// window j inspects tile cursor - 64*j - l); windows are consumed nearest
+// first until a tile with an inclusive prefix is found.  Waiting only on
+// smaller tickets, which belong to workgroups that are already running,
+// guarantees progress.  Returns the exclusive prefix (valid in every lane).
+// On watchdog expiry sets *err and returns what it has (the caller's result
+// is then invalid and the host reports EDPP_ETIMEOUT).
+constexpr int kLookbackWindows = 4;
+
 template <class T, class State>
 __device__ __forceinline__ T lookback(const State& st, int64_t tile, uint32_t spin_limit, int32_t* err) {
     const int lane = lane_id();
     T excl = 0;
-    int64_t cursor = tile - 1;
+    int64_t cursor = tile - 1;  // nearest predecessor not yet accounted for
     uint32_t spins = 0;
     while (true) {
-        int64_t p = cursor - lane;
-        uint32_t flag;
-        T val;
-        if (p >= 0) {
-            st.read(p, flag, val);
-        } else {
-            flag = kFlagPrefix;
-            val = 0;
+        uint32_t flag[kLookbackWindows];
+        T val[kLookbackWindows];
+        #pragma unroll
+        for (int j = 0; j < kLookbackWindows; ++j) {
+            int64_t p = cursor - (int64_t)kWave * j - lane;
+            if (p >= 0) {
+                st.read(p, flag[j], val[j]);
+            } else {
+                flag[j] = kFlagPrefix;  // before tile 0: prefix 0
+                val[j] = 0;
+            }
         }
-        uint64_t pm = ballot(flag == kFlagPrefix);
-        uint64_t zm = ballot(flag == kFlagNone);
-        uint64_t upto = pm ? (pm ^ (pm - 1)) : ~0ull;  // lanes 0..first prefix lane
-        if (zm & upto) {
+        bool stalled = false;
+        #pragma unroll
+        for (int j = 0; j < kLookbackWindows; ++j) {
+            uint64_t pm = ballot(flag[j] == kFlagPrefix);
+            uint64_t zm = ballot(flag[j] == kFlagNone);
+            uint64_t upto = pm ? (pm ^ (pm - 1)) : ~0ull;  // lanes 0..first prefix lane
+            if (zm & upto) {
+                stalled = true;
+                break;
+            }
+            excl += wave_sum(((upto >> lane) & 1ull) ? val[j] : T(0));
+            if (pm) return excl;
+            cursor -= kWave;
+        }
+        if (stalled) {
             if (++spins > spin_limit) {
                 if (lane == 0) st_relaxed(err, (int32_t)EDPP_ETIMEOUT);
                 return excl;
             }
             __builtin_amdgcn_s_sleep(1);
-            continue;
         }
-        T c = ((upto >> lane) & 1ull) ? val : T(0);
-        excl += wave_sum(c);
-        if (pm) return excl;
-        cursor -= kWave;
     }
 }
 
