@@ -68,6 +68,23 @@ TUNING_FIELDS = ("gs_unroll", "gs_random_nt", "scan_vpt", "split_ipt", "split_ti
 _SIGS["edpp_set_tuning"] = [c_vp, ctypes.POINTER(Tuning)]
 _SIGS["edpp_get_tuning"] = [c_vp, ctypes.POINTER(Tuning)]
 
+JOB_SCAN, JOB_SPLIT, JOB_GATHER = 0, 1, 2
+
+
+class Job(ctypes.Structure):
+    """edpp_job of include/edpp_batch.h."""
+    _fields_ = [("kind", c_i32), ("elem_bytes", c_i32), ("n", c_i64), ("buckets", c_i32), ("reserved", c_i32),
+                ("in_", c_vp), ("out", c_vp), ("idx", c_vp), ("vals_in", c_vp), ("vals_out", c_vp), ("aux", c_vp)]
+
+
+_SIGS["edpp_batch_create"] = [c_vp, ctypes.POINTER(Job), c_i32, ctypes.POINTER(c_vp)]
+_SIGS["edpp_batch_run"] = [c_vp, c_dp]
+_SIGS["edpp_batch_run_graph"] = [c_vp, c_dp]
+_SIGS["edpp_batch_destroy"] = [c_vp]
+_SIGS["edpp_batch_set_occupancy"] = [c_vp, c_i32]
+_SIGS["edpp_batch_info"] = [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)]
+_SIGS["edpp_batch_run_plain"] = [c_vp, ctypes.POINTER(Job), c_i32, c_dp]
+
 _RESTYPES = {"edpp_strerror": ctypes.c_char_p, "edpp_last_error": ctypes.c_char_p, "edpp_ctx_stream": c_vp}
 
 
@@ -228,3 +245,51 @@ class Context:
         name = "edpp_gen_split_keys_i64" if wide else "edpp_gen_split_keys_i32"
         self._check(getattr(self._L, name)(self.h, _ptr(keys), _ptr(vals), _numel(keys) if n is None else n,
                                            int(buckets), int(dist), seed & (2**64 - 1)))
+
+
+def make_job(kind, n, src, dst, elem_bytes=4, idx=None, buckets=0, vals_in=None, vals_out=None, aux=None):
+    """One batch job descriptor; buffers as for the single-call methods."""
+    return Job(int(kind), int(elem_bytes), int(n), int(buckets), 0, _ptr(src), _ptr(dst), _ptr(idx), _ptr(vals_in),
+               _ptr(vals_out), _ptr(aux))
+
+
+class Batch:
+    """A planned batch: one persistent launch per run (include/edpp_batch.h)."""
+
+    def __init__(self, ctx, jobs):
+        self.ctx = ctx
+        self._jobs = (Job * len(jobs))(*jobs)
+        h = c_vp()
+        ctx._check(ctx._L.edpp_batch_create(ctx.h, self._jobs, len(jobs), ctypes.byref(h)))
+        self.h = h
+
+    def run(self, timed=False, graph=False):
+        ms = ctypes.c_double(0.0)
+        f = self.ctx._L.edpp_batch_run_graph if graph else self.ctx._L.edpp_batch_run
+        self.ctx._check(f(self.h, ctypes.byref(ms) if timed else None))
+        return ms.value if timed else None
+
+    def set_occupancy(self, wgs_per_cu):
+        self.ctx._check(self.ctx._L.edpp_batch_set_occupancy(self.h, int(wgs_per_cu)))
+
+    def info(self):
+        a, b, c = c_i64(), c_i64(), c_i64()
+        self.ctx._check(self.ctx._L.edpp_batch_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"tiles": a.value, "hist_tiles": b.value, "hscan_tiles": c.value}
+
+    def run_plain(self, timed=False):
+        ms = ctypes.c_double(0.0)
+        self.ctx._check(self.ctx._L.edpp_batch_run_plain(self.ctx.h, self._jobs, len(self._jobs),
+                                                         ctypes.byref(ms) if timed else None))
+        return ms.value if timed else None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx._L.edpp_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -55,20 +55,17 @@ int scan_state_acquire(edpp_ctx* c, int64_t tiles, uint32_t* epoch_out) {
         if (c->scan_status) {
             EDPP_HIP(c, hipStreamSynchronize(c->stream));
             hipFree(c->scan_status);
-            hipFree(c->scan_flags);
-            hipFree(c->scan_vals);
+            hipFree(c->scan_status64);
             c->scan_status = nullptr;
-            c->scan_flags = nullptr;
-            c->scan_vals = nullptr;
+            c->scan_status64 = nullptr;
         }
         if (hipMalloc(&c->scan_status, cap * sizeof(uint64_t)) != hipSuccess ||
-            hipMalloc(&c->scan_flags, cap * sizeof(uint32_t)) != hipSuccess ||
-            hipMalloc(&c->scan_vals, cap * 2 * sizeof(uint64_t)) != hipSuccess) {
+            hipMalloc(&c->scan_status64, cap * 2 * sizeof(uint64_t)) != hipSuccess) {
             c->scan_cap_tiles = 0;
             return fail(c, EDPP_ENOMEM, "look-back state for %lld tiles", (long long)cap);
         }
         EDPP_HIP(c, hipMemsetAsync(c->scan_status, 0, cap * sizeof(uint64_t), c->stream));
-        EDPP_HIP(c, hipMemsetAsync(c->scan_flags, 0, cap * sizeof(uint32_t), c->stream));
+        EDPP_HIP(c, hipMemsetAsync(c->scan_status64, 0, cap * 2 * sizeof(uint64_t), c->stream));
         EDPP_HIP(c, hipMemsetAsync(c->tickets, 0, 2 * sizeof(uint32_t), c->stream));
         c->scan_cap_tiles = cap;
         c->epoch = 1;
@@ -76,7 +73,7 @@ int scan_state_acquire(edpp_ctx* c, int64_t tiles, uint32_t* epoch_out) {
     if (c->epoch >= (1u << 30) - 1) {
         // epochs are 30-bit: clear the state once per 2^30 calls
         EDPP_HIP(c, hipMemsetAsync(c->scan_status, 0, c->scan_cap_tiles * sizeof(uint64_t), c->stream));
-        EDPP_HIP(c, hipMemsetAsync(c->scan_flags, 0, c->scan_cap_tiles * sizeof(uint32_t), c->stream));
+        EDPP_HIP(c, hipMemsetAsync(c->scan_status64, 0, c->scan_cap_tiles * 2 * sizeof(uint64_t), c->stream));
         EDPP_HIP(c, hipMemsetAsync(c->tickets, 0, 2 * sizeof(uint32_t), c->stream));
         c->epoch = 1;
     }
@@ -201,8 +198,7 @@ int edpp_ctx_destroy(edpp_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     hipFree(c->ws);
     hipFree(c->scan_status);
-    hipFree(c->scan_flags);
-    hipFree(c->scan_vals);
+    hipFree(c->scan_status64);
     hipFree(c->tickets);
     hipFree(c->d_err);
     if (c->ev_start) hipEventDestroy(c->ev_start);

@@ -21,8 +21,7 @@ struct edpp_ctx {
     // persistent look-back state of the single-pass scans: tile status words
     // tagged with a per-call epoch, plus two ticket counters used alternately
     uint64_t* scan_status = nullptr;    // u32 scans: packed {epoch, flag, value}
-    uint32_t* scan_flags = nullptr;     // u64 scans: {epoch, flag}
-    uint64_t* scan_vals = nullptr;      // u64 scans: [tile][0]=aggregate, [tile][1]=inclusive
+    uint64_t* scan_status64 = nullptr;  // u64 scans: two tagged words per tile
     int64_t scan_cap_tiles = 0;
     uint32_t* tickets = nullptr;        // [2]
     int32_t* d_err = nullptr;           // device error word written by watchdogs

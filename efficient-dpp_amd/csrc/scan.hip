@@ -216,8 +216,7 @@ template <> LookbackState32 make_state<uint32_t>(edpp_ctx* c, uint32_t epoch) {
 }
 template <> LookbackState64 make_state<uint64_t>(edpp_ctx* c, uint32_t epoch) {
     LookbackState64 s;
-    s.flags = c->scan_flags;
-    s.vals = c->scan_vals;
+    s.status = c->scan_status64;
     s.epoch = epoch;
     return s;
 }

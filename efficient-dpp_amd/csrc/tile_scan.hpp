@@ -54,6 +54,32 @@ struct TileScan {
         }
     }
 
+    // Element-wise agent-scope atomic accesses: for data written by other
+    // workgroups of the same launch, which may sit stale in another XCD's L2.
+    __device__ __forceinline__ void load_coherent(const T* in, int64_t base, int64_t n) {
+        const int t = threadIdx.x;
+        #pragma unroll
+        for (int j = 0; j < VPT; ++j)
+            #pragma unroll
+            for (int k = 0; k < EPV; ++k) {
+                int64_t e = base + (int64_t)j * SUB + (int64_t)t * EPV + k;
+                v[j][k] = e < n ? ld_relaxed(in + e) : T(0);
+            }
+    }
+    __device__ __forceinline__ void store_coherent(T* out, int64_t base, int64_t n, T prefix) const {
+        const int t = threadIdx.x;
+        #pragma unroll
+        for (int j = 0; j < VPT; ++j) {
+            T x = prefix + off[j];
+            #pragma unroll
+            for (int k = 0; k < EPV; ++k) {
+                int64_t e = base + (int64_t)j * SUB + (int64_t)t * EPV + k;
+                if (e < n) st_relaxed(out + e, x);
+                x += v[j][k];
+            }
+        }
+    }
+
     // Block-wide scan of the tile; contains one __syncthreads.
     __device__ __forceinline__ void scan(Smem& sm) {
         const int lane = lane_id();
@@ -138,20 +164,25 @@ struct LookbackState32 {
     }
 };
 
-// 64-bit values: a flag word {epoch:30 | flag:2} per tile published with release
-// after the value, which sits in a separate slot per flag kind (no torn reads).
+// 64-bit values: two 64-bit words per tile, each {epoch:30 | flag:2 | 32 bits
+// of the value} (high half first).  Each word is single-copy atomic and
+// carries its own tag, so a reader accepts the pair only when both words show
+// this call's epoch and the same flag; a half-updated pair reads as "not yet"
+// and is polled again.  No fence is needed on either side.
 struct LookbackState64 {
-    uint32_t* flags;
-    uint64_t* vals;  // [2*t] aggregate, [2*t+1] inclusive prefix
+    uint64_t* status;  // [2*t] high half, [2*t+1] low half
     uint32_t epoch;
     __device__ __forceinline__ void publish(int64_t t, uint32_t flag, uint64_t value) const {
-        st_relaxed(vals + 2 * t + (flag == kFlagPrefix ? 1 : 0), value);
-        st_release(flags + t, (epoch << 2) | flag);
+        const uint64_t tag = ((uint64_t)epoch << 34) | ((uint64_t)flag << 32);
+        st_relaxed(status + 2 * t, (uint64_t)(tag | (value >> 32)));
+        st_relaxed(status + 2 * t + 1, (uint64_t)(tag | (value & 0xFFFFFFFFull)));
     }
     __device__ __forceinline__ void read(int64_t t, uint32_t& flag, uint64_t& value) const {
-        uint32_t f = ld_acquire(flags + t);
-        flag = (f >> 2) == epoch ? (f & 3u) : kFlagNone;
-        value = flag == kFlagNone ? 0 : ld_relaxed(vals + 2 * t + (flag == kFlagPrefix ? 1 : 0));
+        const uint64_t hi = ld_relaxed(status + 2 * t);
+        const uint64_t lo = ld_relaxed(status + 2 * t + 1);
+        const bool live = (uint32_t)(hi >> 34) == epoch && (hi >> 32) == (lo >> 32);
+        flag = live ? (uint32_t)(hi >> 32) & 3u : kFlagNone;
+        value = (hi << 32) | (lo & 0xFFFFFFFFull);
     }
 };
 
