@@ -85,6 +85,32 @@ _SIGS["edpp_batch_set_occupancy"] = [c_vp, c_i32]
 _SIGS["edpp_batch_info"] = [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)]
 _SIGS["edpp_batch_run_plain"] = [c_vp, ctypes.POINTER(Job), c_i32, c_dp]
 
+OP_SCAN, OP_GATHER, OP_SCATTER, OP_SPLIT = 0, 1, 2, 3
+NUM_FEATURES, HIDDEN, MAX_CANDIDATES = 12, 64, 15
+
+
+class Call(ctypes.Structure):
+    """edpp_call of include/edpp_learn.h."""
+    _fields_ = [("op", c_i32), ("elem_bytes", c_i32), ("n", c_i64), ("buckets", c_i32), ("reserved", c_i32),
+                ("in_", c_vp), ("out", c_vp), ("idx", c_vp), ("vals_in", c_vp), ("vals_out", c_vp)]
+
+
+_c_fp = ctypes.POINTER(ctypes.c_float)
+_SIGS["edpp_num_candidates"] = []
+_SIGS["edpp_candidate_info"] = [c_i32, ctypes.POINTER(c_i32), ctypes.c_char_p, c_i32]
+_SIGS["edpp_call_run"] = [c_vp, ctypes.POINTER(Call), c_i32, c_dp]
+_SIGS["edpp_autotune"] = [c_vp, ctypes.POINTER(Call), c_i32, ctypes.POINTER(ctypes.c_double)]
+_SIGS["edpp_call_features"] = [c_vp, ctypes.POINTER(Call), c_vp]
+_SIGS["edpp_mlp_num_params"] = []
+_SIGS["edpp_mlp_create"] = [c_vp, c_u64, ctypes.POINTER(c_vp)]
+_SIGS["edpp_mlp_destroy"] = [c_vp]
+_SIGS["edpp_mlp_get_params"] = [c_vp, _c_fp, c_i64]
+_SIGS["edpp_mlp_set_params"] = [c_vp, _c_fp, c_i64]
+_SIGS["edpp_mlp_forward"] = [c_vp, c_vp, c_vp, c_i32, c_vp, c_vp]
+_SIGS["edpp_mlp_loss_grad"] = [c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp]
+_SIGS["edpp_mlp_train"] = [c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, ctypes.c_float, _c_fp]
+_SIGS["edpp_ctx_set_model"] = [c_vp, c_vp]
+
 _RESTYPES = {"edpp_strerror": ctypes.c_char_p, "edpp_last_error": ctypes.c_char_p, "edpp_ctx_stream": c_vp}
 
 
@@ -190,6 +216,24 @@ class Context:
         self._check(self._L.edpp_get_tuning(self.h, ctypes.byref(t)))
         return {k: getattr(t, k) for k in TUNING_FIELDS}
 
+    # -- schedules, autotuning and the cost model (include/edpp_learn.h)
+    def call_run(self, call, cand=-1, timed=False):
+        ms = ctypes.c_double(0.0)
+        self._check(self._L.edpp_call_run(self.h, ctypes.byref(call), int(cand), ctypes.byref(ms) if timed else None))
+        return ms.value if timed else None
+
+    def autotune(self, call, reps=3):
+        """Median ms of every candidate of the call's family (-1 for other families)."""
+        out = (ctypes.c_double * MAX_CANDIDATES)()
+        self._check(self._L.edpp_autotune(self.h, ctypes.byref(call), int(reps), out))
+        return list(out)
+
+    def call_features(self, call, d_features):
+        self._check(self._L.edpp_call_features(self.h, ctypes.byref(call), _ptr(d_features)))
+
+    def set_model(self, mlp):
+        self._check(self._L.edpp_ctx_set_model(self.h, mlp.h if mlp is not None else None))
+
     def sync(self):
         self._check(self._L.edpp_ctx_sync(self.h))
 
@@ -286,6 +330,71 @@ class Batch:
     def close(self):
         if getattr(self, "h", None):
             self.ctx._L.edpp_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_call(op, n, src, dst, elem_bytes=4, idx=None, buckets=0, vals_in=None, vals_out=None):
+    return Call(int(op), int(elem_bytes), int(n), int(buckets), 0, _ptr(src), _ptr(dst), _ptr(idx), _ptr(vals_in),
+                _ptr(vals_out))
+
+
+def candidates():
+    """[(family, name)] of every schedule candidate."""
+    L = lib()
+    out = []
+    for k in range(L.edpp_num_candidates()):
+        fam = c_i32()
+        name = ctypes.create_string_buffer(96)
+        L.edpp_candidate_info(k, ctypes.byref(fam), name, 96)
+        out.append((fam.value, name.value.decode()))
+    return out
+
+
+class MLP:
+    """The FP32 cost model 12 -> 64 -> 64 -> 15 trained by the library's own HIP kernels."""
+
+    def __init__(self, ctx, seed=0):
+        self.ctx = ctx
+        h = c_vp()
+        ctx._check(ctx._L.edpp_mlp_create(ctx.h, seed & (2**64 - 1), ctypes.byref(h)))
+        self.h = h
+        self.nparams = ctx._L.edpp_mlp_num_params()
+
+    def get_params(self):
+        import numpy as np
+        out = np.empty(self.nparams, np.float32)
+        self.ctx._check(self.ctx._L.edpp_mlp_get_params(self.h, out.ctypes.data_as(_c_fp), self.nparams))
+        return out
+
+    def set_params(self, p):
+        import numpy as np
+        p = np.ascontiguousarray(p, np.float32)
+        self.ctx._check(self.ctx._L.edpp_mlp_set_params(self.h, p.ctypes.data_as(_c_fp), p.size))
+
+    def forward(self, x, mask, probs=None, choice=None):
+        rows = x.shape[0]
+        self.ctx._check(self.ctx._L.edpp_mlp_forward(self.h, _ptr(x), _ptr(mask), rows, _ptr(probs), _ptr(choice)))
+
+    def loss_grad(self, x, target, mask, loss, grad):
+        self.ctx._check(self.ctx._L.edpp_mlp_loss_grad(self.h, _ptr(x), _ptr(target), _ptr(mask), x.shape[0],
+                                                       _ptr(loss), _ptr(grad)))
+
+    def train(self, x, target, mask, epochs, lr=1e-2):
+        import numpy as np
+        losses = np.zeros(max(epochs, 1), np.float32)
+        self.ctx._check(self.ctx._L.edpp_mlp_train(self.h, _ptr(x), _ptr(target), _ptr(mask), x.shape[0],
+                                                   int(epochs), float(lr), losses.ctypes.data_as(_c_fp)))
+        return losses[:epochs]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx._L.edpp_mlp_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -280,7 +280,22 @@ static int gs_common(edpp_ctx* c, const T* in, T* out, const int32_t* idx, int64
     if (passes < 0) return fail(c, EDPP_EINVAL, "pass count %d < 0", passes);
     if (n > 0 && (const void*)in == (const void*)out) return fail(c, EDPP_EINVAL, "in and out must differ");
     int rc = timing_begin(c, ms);
-    if (!rc) rc = launch_gather<T>(c, in, out, idx, n, passes, scatter);
+    if (!rc) {
+        if (passes == 0 && c->model && n > 0) {
+            edpp_call q = {};
+            q.op = scatter ? EDPP_OP_SCATTER : EDPP_OP_GATHER;
+            q.elem_bytes = (int32_t)sizeof(T);
+            q.n = n;
+            q.in = in;
+            q.out = out;
+            q.idx = idx;
+            int k = 0;
+            rc = model_choose(c, q, &k);
+            if (!rc) rc = run_candidate(c, q, k, nullptr);
+        } else {
+            rc = launch_gather<T>(c, in, out, idx, n, passes, scatter);
+        }
+    }
     if (!rc) rc = timing_end(c, ms);
     return rc;
 }
@@ -314,7 +329,21 @@ static int scan_common(edpp_ctx* c, const T* in, T* out, int64_t n, int scheme, 
     if (n > 0 && (!in || !out)) return fail(c, EDPP_EINVAL, "null buffer");
     if (scheme < EDPP_SCAN_AUTO || scheme > EDPP_SCAN_SSA) return fail(c, EDPP_EINVAL, "unknown scheme %d", scheme);
     int rc = timing_begin(c, ms);
-    if (!rc) rc = launch_scan<T>(c, in, out, n, scheme, d_total);
+    if (!rc) {
+        if (scheme == EDPP_SCAN_AUTO && c->model && n > 0) {
+            edpp_call q = {};
+            q.op = EDPP_OP_SCAN;
+            q.elem_bytes = (int32_t)sizeof(T);
+            q.n = n;
+            q.in = in;
+            q.out = out;
+            int k = 0;
+            rc = model_choose(c, q, &k);
+            if (!rc) rc = run_candidate(c, q, k, d_total);
+        } else {
+            rc = launch_scan<T>(c, in, out, n, scheme, d_total);
+        }
+    }
     if (!rc) rc = timing_end(c, ms);
     return rc;
 }
@@ -358,7 +387,24 @@ static int split_common(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, 
     if (strategy < EDPP_SPLIT_AUTO || strategy > EDPP_SPLIT_SINGLE)
         return fail(c, EDPP_EINVAL, "unknown split strategy %d", strategy);
     int rc = timing_begin(c, ms);
-    if (!rc) rc = launch_split<K>(c, kin, kout, vin, vout, d_start, n, buckets, strategy);
+    if (!rc) {
+        if (strategy == EDPP_SPLIT_AUTO && c->model && n > 0) {
+            edpp_call q = {};
+            q.op = EDPP_OP_SPLIT;
+            q.elem_bytes = (int32_t)sizeof(K);
+            q.n = n;
+            q.buckets = buckets;
+            q.in = kin;
+            q.out = kout;
+            q.vals_in = vin;
+            q.vals_out = vout;
+            int k = 0;
+            rc = model_choose(c, q, &k);
+            if (!rc) rc = run_candidate(c, q, k, d_start);
+        } else {
+            rc = launch_split<K>(c, kin, kout, vin, vout, d_start, n, buckets, strategy);
+        }
+    }
     if (!rc) rc = timing_end(c, ms);
     return rc;
 }

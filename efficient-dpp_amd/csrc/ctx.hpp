@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 #include "edpp.h"
+#include "edpp_learn.h"
 #include "edpp_tune.h"
 
 struct edpp_ctx {
@@ -29,6 +30,7 @@ struct edpp_ctx {
     uint32_t spin_limit = 1u << 22;     // look-back watchdog (polls before giving up, seconds)
 
     edpp_tuning tune = default_tuning();
+    edpp_mlp* model = nullptr;          // when set, AUTO calls ask the cost model
     std::string last_error;
 
     static edpp_tuning default_tuning() {
@@ -83,6 +85,11 @@ int launch_gen_u64(edpp_ctx* c, uint64_t* out, int64_t n, uint64_t seed, uint64_
 int launch_gen_perm(edpp_ctx* c, int32_t* idx, int64_t n, uint64_t seed, int kind);
 template <class K>
 int launch_gen_split_keys(edpp_ctx* c, K* keys, int32_t* vals, int64_t n, int buckets, int dist, uint64_t seed);
+
+// schedule candidates and the cost model (learn.hip)
+int validate_call(edpp_ctx* c, const edpp_call& q);
+int model_choose(edpp_ctx* c, const edpp_call& q, int* cand);
+int run_candidate(edpp_ctx* c, const edpp_call& q, int cand, void* aux);
 
 // schedule defaults (overridden by the tuner / cost model)
 int default_passes(edpp_ctx* c, int64_t n, int elem_bytes, bool scatter);
