@@ -37,6 +37,11 @@ def parse():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--suite", action="store_true", help="measure every primitive too")
+    p.add_argument("--batched", action="store_true", help="10,000-job mix: one persistent launch vs per-call")
+    p.add_argument("--learned", action="store_true", help="train the schedule model on autotuner timings")
+    p.add_argument("--train-calls", type=int, default=2000)
+    p.add_argument("--test-calls", type=int, default=500)
+    p.add_argument("--db-out", default=None, help="write the autotuner database (JSON lines) here")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-sample", type=int, default=1 << 26, help="elements in the CPU baseline sample")
     p.add_argument("--passes", type=int, default=0, help="pass count of the default step (0 = library schedule)")
@@ -203,6 +208,13 @@ def main():
     del dst2
     if args.suite:
         line["suite"] = suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs)
+    if args.batched or args.learned:
+        del src, dst, idx
+        torch.cuda.empty_cache()
+    if args.batched:
+        line["batched"] = batched(ctx, edpp, torch)
+    if args.learned:
+        line["learned"] = learned(ctx, edpp, torch, args)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args)
     s = json.dumps(line)
@@ -268,6 +280,87 @@ def suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs):
     del ko, vo, b
     ctx.gen_i32(src, 0xDA7A)
     return res
+
+
+def batched(ctx, edpp, torch):
+    """C4: 10,000 jobs (scan / 256-bucket split / gather, int32 or int64,
+    log-uniform lengths rescaled to 2^28 elements in total): one persistent
+    launch (and its hipGraph replay) against one library call per job."""
+    import importlib
+    jm = importlib.import_module("efficient-dpp_amd.jobmix")
+    plan, scale = jm.mix_plan(10000, 1 << 28, seed=4)
+    jobsets, bufs, arena = jm.build_mix(edpp, ctx, torch, plan, outputs=2)
+    b = edpp.Batch(ctx, jobsets[0])
+    plain = edpp.Batch(ctx, jobsets[1])
+
+    def med(f, k):
+        f()
+        return float(np.median([f() for _ in range(k)]))
+
+    t_run = med(lambda: b.run(timed=True), 5)
+    t_graph = med(lambda: b.run(timed=True, graph=True), 5)
+    t_plain = med(lambda: plain.run_plain(timed=True), 3)
+    bad = 0
+    for bb in bufs:
+        same = torch.equal(bb["out0"], bb["out1"])
+        if bb["plan"]["kind"] == 1:
+            same = same and torch.equal(bb["vout0"], bb["vout1"]) and torch.equal(bb["start0"], bb["start1"])
+        if bb["plan"]["kind"] == 0:
+            same = same and torch.equal(bb["total0"], bb["total1"])
+        bad += 0 if same else 1
+    elems = sum(p["n"] for p in plan)
+    nbytes = 0
+    for p in plan:
+        w = p["width"]
+        nbytes += p["n"] * (2 * w if p["kind"] == 0 else (2 * (w + 4) if p["kind"] == 1 else 4 + 2 * w))
+    best = min(t_run, t_graph)
+    out = {"jobs": len(plan), "elements": elems, "length_scale": scale, "tiles": b.info()["tiles"],
+           "batched_ms": round(t_run, 4), "graph_ms": round(t_graph, 4), "plain_per_call_ms": round(t_plain, 3),
+           "speedup": round(t_plain / best, 2), "batched_gbs": round(nbytes / (best * 1e-3) / 1e9, 1),
+           "jobs_differing_from_per_call": bad}
+    print("batched", out, file=sys.stderr, flush=True)
+    b.close()
+    plain.close()
+    del bufs, arena, jobsets
+    torch.cuda.empty_cache()
+    return out
+
+
+def learned(ctx, edpp, torch, args):
+    """C5: exhaustive autotune of seeded calls, HIP training of the cost
+    model, held-out regret against exhaustive search and end-to-end rate."""
+    import importlib
+    tn = importlib.import_module("efficient-dpp_amd.tuner")
+    t0 = time.time()
+    ws = tn.Workspace(edpp, ctx, torch, N_FULL)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    train_recs = tn.measure(ws, tn.sample_specs(args.train_calls, seed=2024), reps=3, log=log)
+    test_recs = tn.measure(ws, tn.sample_specs(args.test_calls, seed=777), reps=3, log=log)
+    if args.db_out:
+        tn.save_db(args.db_out, train_recs + test_recs)
+    t1 = time.time()
+    mlp, losses = tn.train(edpp, ctx, torch, train_recs, epochs=1500, lr=3e-3)
+    t2 = time.time()
+    pred = tn.predict(ctx, torch, mlp, test_recs)
+    model = tn.regret(test_recs, pred)
+    default = tn.regret(test_recs, tn.default_choices(test_recs))
+    train_fit = tn.regret(train_recs, tn.predict(ctx, torch, mlp, train_recs))
+    specs = [r["spec"] for r in test_recs]
+    ctx.set_model(None)
+    e2e_default = tn.end_to_end(ws, ctx, specs, False)
+    ctx.set_model(mlp)
+    e2e_model = tn.end_to_end(ws, ctx, specs, True)
+    ctx.set_model(None)
+    best_bytes = sum(tn.call_bytes(s) for s in specs)
+    out = {"train_calls": len(train_recs), "test_calls": len(test_recs), "autotune_s": round(t1 - t0, 1),
+           "train_s": round(t2 - t1, 2), "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+           "regret_model": model, "regret_default": default, "regret_model_on_train": train_fit,
+           "exhaustive_gbs": round(best_bytes / (model["ms_best"] * 1e-3) / 1e9, 1),
+           "end_to_end_model": e2e_model, "end_to_end_default": e2e_default,
+           "best_candidate_histogram": tn.summarize_times(train_recs)}
+    print("learned", json.dumps(out), file=sys.stderr, flush=True)
+    mlp.close()
+    return out
 
 
 if __name__ == "__main__":

@@ -66,6 +66,20 @@ def build(verbose=False, jobs=None):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the C++ driver of the reference-shaped API (tests/cpp), linked against the library
+    drv_src = os.path.join(REPO, "tests", "cpp", "test_primitives.cpp")
+    if os.path.exists(drv_src):
+        drv = os.path.join(REPO, "tests", "cpp", "test_primitives")
+        obj = os.path.join(BUILD, "test_primitives.o")
+        for cmd in ([HIPCC, "-x", "hip", "-O2", "-std=c++17", "-Wno-unused-result", f"--offload-arch={ARCH}",
+                     "-I", INCLUDE, "-c", drv_src, "-o", obj],
+                    [HIPCC, f"--offload-arch={ARCH}", obj, "-L", PKG_DIR, "-ledpp",
+                     "-Wl,-rpath,$ORIGIN/../../efficient-dpp_amd", "-o", drv]):
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(f"driver build failed:\n{r.stdout}\n{r.stderr}")
     return LIB
 
 

@@ -108,18 +108,42 @@ int timing_end(edpp_ctx* c, double* ms) {
 }
 
 // ---------------------------------------------------------------- schedules
-// Defaults used when the caller passes 0 / AUTO and no tuned schedule exists.
+// Built-in schedules for AUTO calls when no cost model is set, read off the
+// autotuner's database on MI355X (2,000 calls, n in [2^16, 2^28]).
 int default_passes(edpp_ctx* c, int64_t n, int elem_bytes, bool scatter) {
-    (void)c; (void)n; (void)elem_bytes; (void)scatter;
+    (void)c;
+    if (!scatter) return 1;  // random reads gain nothing from re-reading the index
+    // random writes: a pass per quarter of the Infinity Cache keeps the
+    // written slice resident, so lines leave it whole
+    const int64_t bytes = n * (int64_t)elem_bytes;
+    if (bytes >= (512ll << 20)) return 4;
+    if (bytes >= (64ll << 20)) return 2;
     return 1;
 }
 int default_scan_scheme(edpp_ctx* c, int64_t n, int elem_bytes) {
-    (void)c; (void)n; (void)elem_bytes;
+    (void)c;
+    if (n < (1ll << 19)) return EDPP_SCAN_LOOKBACK;
+    if (elem_bytes == 8 || n < (1ll << 23)) return EDPP_SCAN_RTS;
     return EDPP_SCAN_LOOKBACK;
+}
+void default_scan_shape(edpp_ctx* c, int64_t n, int* block, int* vpt) {
+    (void)c;
+    if (n < (1ll << 21)) {
+        *block = 256;
+        *vpt = 8;
+    } else {
+        *block = 512;
+        *vpt = 16;
+    }
 }
 int default_split_strategy(edpp_ctx* c, int64_t n, int buckets) {
     (void)c; (void)n; (void)buckets;
     return EDPP_SPLIT_WG;
+}
+void default_split_shape(edpp_ctx* c, int buckets, int key_bytes, int* ipt, int* mult) {
+    (void)c;
+    *ipt = (buckets == 256 && key_bytes == 4) ? 16 : 8;
+    *mult = 16;
 }
 
 }  // namespace edpp

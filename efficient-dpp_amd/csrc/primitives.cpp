@@ -1,0 +1,120 @@
+// primitives.cpp — the reference's C++ host API (include/edpp_primitives.hpp)
+// over the C ABI.  Every function validates its arguments the way the
+// reference does, runs synchronously on the process-wide context and returns
+// device milliseconds.
+#include <mutex>
+#include "edpp_primitives.hpp"
+
+namespace edpp {
+
+namespace {
+std::once_flag g_once;
+edpp_ctx* g_ctx = nullptr;
+
+double ms_or_fail(int rc, double ms) { return rc == EDPP_OK ? ms : -1.0; }
+}  // namespace
+
+edpp_ctx* context() {
+    std::call_once(g_once, [] {
+        if (edpp_ctx_create(0, nullptr, &g_ctx) != EDPP_OK) g_ctx = nullptr;
+    });
+    return g_ctx;
+}
+
+double gather(const int* d_in, int* d_out, int length, const int* d_loc, int localSize, int gridSize, int pass) {
+    (void)localSize;
+    (void)gridSize;
+    edpp_ctx* c = context();
+    if (!c) return -1.0;
+    double ms = 0.0;
+    return ms_or_fail(edpp_gather_i32(c, d_in, d_out, d_loc, length, pass < 1 ? 1 : pass, &ms), ms);
+}
+
+double scatter(const int* d_in, int* d_out, int length, const int* d_loc, int localSize, int gridSize, int pass) {
+    (void)localSize;
+    (void)gridSize;
+    edpp_ctx* c = context();
+    if (!c) return -1.0;
+    double ms = 0.0;
+    return ms_or_fail(edpp_scatter_i32(c, d_in, d_out, d_loc, length, pass < 1 ? 1 : pass, &ms), ms);
+}
+
+double scan_chained(const int* d_in, int* d_out, int length, int localSize, int gridSize, int R, int L) {
+    (void)localSize;
+    (void)gridSize;
+    if (R == 0 && L == 0) return 1;  // opencl/primitives/scanImpl.cpp:19-22
+    edpp_ctx* c = context();
+    if (!c) return -1.0;
+    double ms = 0.0;
+    return ms_or_fail(edpp_scan_exclusive_i32(c, d_in, d_out, length, EDPP_SCAN_LOOKBACK, nullptr, &ms), ms);
+}
+
+double scan_RSS(const int* d_in, int* d_out, unsigned length, int local_size, int grid_size) {
+    (void)local_size;
+    (void)grid_size;
+    edpp_ctx* c = context();
+    if (!c) return -1.0;
+    double ms = 0.0;
+    return ms_or_fail(edpp_scan_exclusive_i32(c, d_in, d_out, length, EDPP_SCAN_RTS, nullptr, &ms), ms);
+}
+
+double scan_RSS_single(const int* d_in, int* d_out, unsigned length) {
+    edpp_ctx* c = context();
+    if (!c) return -1.0;
+    double ms = 0.0;
+    return ms_or_fail(edpp_scan_exclusive_i32(c, d_in, d_out, length, EDPP_SCAN_SSA, nullptr, &ms), ms);
+}
+
+double matrix_scan(const int* d_in, int* d_out, int tile_size, MatrixScanType type, int local_size) {
+    edpp_ctx* c = context();
+    if (!c) return -1.0;
+    double ms = 0.0;
+    return ms_or_fail(edpp_scan_local_i32(c, d_in, d_out, tile_size, local_size, (int)type, EDPP_NET_WAVE, &ms), ms);
+}
+
+namespace {
+
+// KO and KVS_SOA map to the key/value split; KVS_AOS holds (key, value) int
+// pairs, which split as 64-bit elements whose low word is the key.
+double split_common(const int* d_in, int* d_out, int* d_start, int length, int buckets, DataStruc structure,
+                    const int* d_in_values, int* d_out_values, int strategy) {
+    if (structure == KVS_SOA && (!d_in_values || !d_out_values)) return -1;  // splitImpl.cpp:29-34
+    if (structure != KVS_SOA && (d_in_values || d_out_values)) return -1;
+    edpp_ctx* c = context();
+    if (!c) return -1.0;
+    double ms = 0.0;
+    int rc;
+    if (structure == KVS_AOS)
+        rc = edpp_split_i64(c, reinterpret_cast<const int64_t*>(d_in), reinterpret_cast<int64_t*>(d_out), nullptr,
+                            nullptr, d_start, length, buckets, strategy, &ms);
+    else
+        rc = edpp_split_i32(c, d_in, d_out, structure == KVS_SOA ? d_in_values : nullptr,
+                            structure == KVS_SOA ? d_out_values : nullptr, d_start, length, buckets, strategy, &ms);
+    return ms_or_fail(rc, ms);
+}
+
+}  // namespace
+
+double WI_split(const int* d_in, int* d_out, int* d_start, int length, int buckets, DataStruc structure,
+                const int* d_in_values, int* d_out_values, int local_size, int grid_size) {
+    (void)local_size;
+    (void)grid_size;
+    return split_common(d_in, d_out, d_start, length, buckets, structure, d_in_values, d_out_values, EDPP_SPLIT_WI);
+}
+
+double WG_split(const int* d_in, int* d_out, int* d_start, int length, int buckets, ReorderType reorder_type,
+                DataStruc structure, const int* d_in_values, int* d_out_values, int local_size, int grid_size) {
+    (void)local_size;
+    (void)grid_size;
+    // every reorder variant is the stable WG split with its LDS reorder buffer
+    (void)reorder_type;
+    return split_common(d_in, d_out, d_start, length, buckets, structure, d_in_values, d_out_values, EDPP_SPLIT_WG);
+}
+
+double single_split(const int* d_in, int* d_out, int length, int buckets, bool reorder, DataStruc structure) {
+    (void)reorder;
+    if (structure == KVS_SOA) return -1;  // splitImpl.cpp:376-379: SOA not supported
+    return split_common(d_in, d_out, nullptr, length, buckets, structure, nullptr, nullptr, EDPP_SPLIT_SINGLE);
+}
+
+}  // namespace edpp

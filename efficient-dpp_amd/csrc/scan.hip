@@ -284,7 +284,16 @@ int launch_scan(edpp_ctx* c, const T* in, T* out, int64_t n, int scheme, T* d_to
         return EDPP_OK;
     }
     if (!aligned16(in) || !aligned16(out)) return run_unaligned<T>(c, in, out, n, d_total);
-    if (scheme == EDPP_SCAN_AUTO) scheme = default_scan_scheme(c, n, (int)sizeof(T));
+    if (scheme == EDPP_SCAN_AUTO) {
+        scheme = default_scan_scheme(c, n, (int)sizeof(T));
+        if (scheme == EDPP_SCAN_LOOKBACK) {
+            const edpp_tuning saved = c->tune;
+            default_scan_shape(c, n, &c->tune.scan_block, &c->tune.scan_vpt);
+            int rc = launch_scan<T>(c, in, out, n, EDPP_SCAN_LOOKBACK, d_total);
+            c->tune = saved;
+            return rc;
+        }
+    }
     // tile of 4096 (u32) / 2048 (u64) elements for look-back; 8192 / 4096 for
     // the three-kernel schemes, whose partial scan is the serial bottleneck
     switch (scheme) {

@@ -414,7 +414,13 @@ int launch_split(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t
         if (d_start) EDPP_HIP(c, hipMemsetAsync(d_start, 0, sizeof(int32_t) * buckets, c->stream));
         return EDPP_OK;
     }
-    if (strategy == EDPP_SPLIT_AUTO) strategy = default_split_strategy(c, n, buckets);
+    if (strategy == EDPP_SPLIT_AUTO) {
+        const edpp_tuning saved = c->tune;
+        default_split_shape(c, buckets, (int)sizeof(K), &c->tune.split_ipt, &c->tune.split_tile_mult);
+        int rc = launch_split<K>(c, kin, kout, vin, vout, d_start, n, buckets, default_split_strategy(c, n, buckets));
+        c->tune = saved;
+        return rc;
+    }
     if (strategy < EDPP_SPLIT_WG || strategy > EDPP_SPLIT_SINGLE)
         return fail(c, EDPP_EINVAL, "unknown split strategy %d", strategy);
     const bool two = strategy != EDPP_SPLIT_SINGLE && lg > kMaxSplitBitsPerPass;

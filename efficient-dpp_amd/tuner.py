@@ -17,7 +17,8 @@ import numpy as np
 
 OPS = ("scan", "gather", "scatter", "split")
 FAMILY_OF_OP = {0: 0, 1: 1, 2: 1, 3: 2}
-# the built-in schedule of each family when no model is set (the method without the model)
+# one fixed schedule per family: the method without the model (look-back 512x16,
+# one pass, WG split with 16 keys per lane and tiles of 64 elements per bucket)
 DEFAULT_CANDIDATE = {0: 0, 1: 5, 2: 11}
 
 
@@ -117,7 +118,8 @@ def arrays(records, tau=0.02, k=15):
         valid = t > 0
         M[i, valid] = 1.0
         best = t[valid].min()
-        s = np.where(valid, np.exp(-(t / best - 1.0) / tau), 0.0)
+        s = np.zeros(k)
+        s[valid] = np.exp(-(t[valid] / best - 1.0) / tau)
         T[i] = s / s.sum()
     return X, T, M
 
