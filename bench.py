@@ -46,6 +46,8 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=1 << 26, help="elements in the CPU baseline sample")
     p.add_argument("--passes", type=int, default=0, help="pass count of the default step (0 = library schedule)")
     p.add_argument("--out", default=None, help="also write the JSON line to this file")
+    p.add_argument("--profile", default=None, choices=["stream", "gather", "scatter", "scan", "split", "batched"],
+                   help="only run one workload at full size (for rocprofv3), plus one STREAM copy")
     return p.parse_args()
 
 
@@ -130,12 +132,59 @@ def cpu_baseline(args, torch_unused=None):
     }
 
 
+def profile_only(args, ctx, edpp, torch):
+    """One workload at full size, `steps` times, then one STREAM copy of 2^28
+    int32 (the calibration kernel for the byte counters)."""
+    n = N_FULL
+    i32 = torch.int32
+    a = torch.empty(n, dtype=i32, device="cuda")
+    b = torch.empty(n, dtype=i32, device="cuda")
+    ctx.gen_i32(a, 0xDA7A)
+    reps = max(1, args.steps)
+    if args.profile in ("gather", "scatter"):
+        idx = torch.empty(n, dtype=i32, device="cuda")
+        ctx.gen_perm(idx, 0x5EED, edpp.PERM_RANDOM)
+        f = ctx.gather if args.profile == "gather" else ctx.scatter
+        for _ in range(reps):
+            f(a, b, idx, n, args.passes)
+    elif args.profile == "scan":
+        ctx.gen_i32(a, 2024, 15)
+        for _ in range(reps):
+            ctx.scan(a, b, n, edpp.SCAN_LOOKBACK)
+    elif args.profile == "split":
+        v = torch.empty(n, dtype=i32, device="cuda")
+        vo = torch.empty(n, dtype=i32, device="cuda")
+        ctx.gen_split_keys(a, v, 256, edpp.KEYS_UNIFORM, 0x5917)
+        for _ in range(reps):
+            ctx.split(a, b, 256, v, vo, None, n, edpp.SPLIT_AUTO)
+    elif args.profile == "batched":
+        del a, b
+        batched(ctx, edpp, torch)
+        a = torch.empty(n, dtype=i32, device="cuda")
+        b = torch.empty(n, dtype=i32, device="cuda")
+    else:
+        for _ in range(reps):
+            ctx.stream_op(edpp.STREAM_COPY, a, None, b, n)
+    ctx.stream_op(edpp.STREAM_COPY, a, None, b, n)
+    torch.cuda.synchronize()
+    print(json.dumps({"profile": args.profile, "n": n, "reps": reps}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is design-only.",
               file=sys.stderr)
         sys.exit(2)
+    if args.profile:
+        import importlib
+        import torch
+        edpp = importlib.import_module("efficient-dpp_amd")
+        torch.cuda.set_device(0)
+        ctx = edpp.Context(0, stream=torch.cuda.current_stream().cuda_stream)
+        profile_only(args, ctx, edpp, torch)
+        ctx.close()
+        return
     import importlib
     import torch
     edpp = importlib.import_module("efficient-dpp_amd")

@@ -142,7 +142,9 @@ int default_split_strategy(edpp_ctx* c, int64_t n, int buckets) {
 }
 void default_split_shape(edpp_ctx* c, int buckets, int key_bytes, int* ipt, int* mult) {
     (void)c;
-    *ipt = (buckets == 256 && key_bytes == 4) ? 16 : 8;
+    (void)buckets;
+    (void)key_bytes;
+    *ipt = 16;   // wg_ipt() falls back to 8 when the LDS reorder buffer would not fit
     *mult = 16;
 }
 

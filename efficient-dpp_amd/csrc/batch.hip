@@ -121,9 +121,8 @@ __device__ void split_hist_tile(const DJob& J, int64_t t, uint32_t* lds) {
     for (int b = threadIdx.x; b < kSplitWaves * B; b += kBatchBlock) whist[b] = 0;
     __syncthreads();
     const int64_t base = t * S, end = std::min<int64_t>(base + S, J.n);
-    uint32_t bk[IPT], rk[IPT];
-    K key[IPT];
-    rank_subtile<K, IPT>((const K*)J.in, base, end, J.lg, 0, whist + w * B, bk, rk, key);
+    if (J.lg <= kBallotMatchBits) count_subtile<K, IPT, true>((const K*)J.in, base, end, J.lg, 0, whist + w * B);
+    else count_subtile<K, IPT, false>((const K*)J.in, base, end, J.lg, 0, whist + w * B);
     __syncthreads();
     for (int b = threadIdx.x; b < B; b += kBatchBlock) {
         uint32_t a = 0;
@@ -137,7 +136,8 @@ template <class K, int IPT>
 __device__ void split_scatter_tile(const DJob& J, int64_t t, uint32_t* lds) {
     const int B = 1 << J.lg;
     constexpr int S = kBatchBlock * IPT;
-    uint32_t* whist = lds;
+    uint64_t* wmask = reinterpret_cast<uint64_t*>(lds);
+    uint32_t* whist = lds + whist_word_offset(B);
     uint32_t* run = whist + kSplitWaves * B;
     uint32_t* sub = run + B;
     uint32_t* off = sub + B;
@@ -153,21 +153,21 @@ __device__ void split_scatter_tile(const DJob& J, int64_t t, uint32_t* lds) {
         run[b] = r;
         if (t == 0 && J.aux) static_cast<int32_t*>(J.aux)[b] = (int32_t)r;
     }
-    for (int b = threadIdx.x; b < kSplitWaves * B; b += kBatchBlock) whist[b] = 0;
+    for (int b = threadIdx.x; b < kSplitWaves * B; b += kBatchBlock) {
+        whist[b] = 0;
+        wmask[b] = 0ull;
+    }
     __syncthreads();
     const int64_t s = t * S, tend = std::min<int64_t>(s + S, J.n);
     uint32_t bk[IPT], rk[IPT];
     K key[IPT];
-    rank_subtile<K, IPT>((const K*)J.in, s, tend, J.lg, 0, whist + w * B, bk, rk, key);
     int32_t val[IPT];
-    if (kv) {
-        const int64_t wbase = s + (int64_t)w * kWave * IPT;
-        #pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-            int64_t e = wbase + (int64_t)j * kWave + lane;
-            val[j] = e < tend ? J.vin[e] : 0;
-        }
-    }
+    if (J.lg <= kBallotMatchBits)
+        rank_subtile<K, IPT, true>((const K*)J.in, J.vin, s, tend, J.lg, 0, whist + w * B, wmask + w * B, bk, rk, key,
+                                   val);
+    else
+        rank_subtile<K, IPT, false>((const K*)J.in, J.vin, s, tend, J.lg, 0, whist + w * B, wmask + w * B, bk, rk,
+                                    key, val);
     __syncthreads();
     for (int b = threadIdx.x; b < B; b += kBatchBlock) {
         uint32_t a = 0;
@@ -309,7 +309,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const DJob* __restrict__ 
     }
 }
 
-constexpr size_t kBatchLds = 40 * 1024;  // split scatter, 32-bit keys: 7232 + 4096*8 bytes
+constexpr size_t kBatchLds = 48 * 1024;  // split scatter, 32-bit keys: 8192 + 7232 + 4096*8 bytes
 
 }  // namespace
 }  // namespace edpp

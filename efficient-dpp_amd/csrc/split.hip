@@ -22,7 +22,7 @@
 
 namespace edpp {
 
-template <class K, int IPT>
+template <class K, int IPT, bool BALLOT>
 __global__ __launch_bounds__(kSplitBlock) void k_split_hist(const K* __restrict__ kin, SplitGeom g,
                                                             int32_t* __restrict__ his) {
     extern __shared__ uint32_t lds[];
@@ -35,11 +35,8 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_hist(const K* __restrict_
     const int w = threadIdx.x / kWave;
     for (int b = threadIdx.x; b < kSplitWaves * B; b += kSplitBlock) whist[b] = 0;
     __syncthreads();
-    for (int64_t s = tbase; s < tend; s += S) {
-        uint32_t bk[IPT], rk[IPT];
-        K key[IPT];
-        rank_subtile<K, IPT>(kin, s, tend, g.lg, g.shift, whist + w * B, bk, rk, key);
-    }
+    for (int64_t s = tbase; s < tend; s += S)
+        count_subtile<K, IPT, BALLOT>(kin, s, tend, g.lg, g.shift, whist + w * B);
     __syncthreads();
     for (int b = threadIdx.x; b < B; b += kSplitBlock) {
         uint32_t a = 0;
@@ -48,7 +45,7 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_hist(const K* __restrict_
     }
 }
 
-template <class K, int IPT, bool KV>
+template <class K, int IPT, bool KV, bool BALLOT>
 __global__ __launch_bounds__(kSplitBlock) void k_split_scatter(const K* __restrict__ kin, K* __restrict__ kout,
                                                                const int32_t* __restrict__ vin,
                                                                int32_t* __restrict__ vout, SplitGeom g,
@@ -56,8 +53,9 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_scatter(const K* __restri
     extern __shared__ uint32_t lds[];
     const int B = 1 << g.lg;
     constexpr int S = kSplitBlock * IPT;
-    uint32_t* whist = lds;                       // [waves][B]
-    uint32_t* run = whist + kSplitWaves * B;     // [B] next output position of each bucket
+    uint64_t* wmask = reinterpret_cast<uint64_t*>(lds);  // [waves][B], zero between items
+    uint32_t* whist = lds + whist_word_offset(B);        // [waves][B]
+    uint32_t* run = whist + kSplitWaves * B;             // [B] next output position of each bucket
     uint32_t* sub = run + B;                     // [B] sub-tile bucket counts
     uint32_t* off = sub + B;                     // [B] sub-tile bucket offsets
     uint32_t* wtmp = off + B;                    // [waves]
@@ -70,21 +68,15 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_scatter(const K* __restri
     const int w = threadIdx.x / kWave;
     const int lane = lane_id();
     for (int b = threadIdx.x; b < B; b += kSplitBlock) run[b] = (uint32_t)scanned[(int64_t)b * g.tiles + tile];
+    for (int b = threadIdx.x; b < kSplitWaves * B; b += kSplitBlock) wmask[b] = 0ull;
     for (int64_t s = tbase; s < tend; s += S) {
         for (int b = threadIdx.x; b < kSplitWaves * B; b += kSplitBlock) whist[b] = 0;
         __syncthreads();
         uint32_t bk[IPT], rk[IPT];
         K key[IPT];
-        rank_subtile<K, IPT>(kin, s, tend, g.lg, g.shift, whist + w * B, bk, rk, key);
         int32_t val[IPT];
-        if (KV) {
-            const int64_t wbase = s + (int64_t)w * kWave * IPT;
-            #pragma unroll
-            for (int j = 0; j < IPT; ++j) {
-                int64_t e = wbase + (int64_t)j * kWave + lane;
-                val[j] = e < tend ? vin[e] : 0;
-            }
-        }
+        rank_subtile<K, IPT, BALLOT>(kin, KV ? vin : nullptr, s, tend, g.lg, g.shift, whist + w * B, wmask + w * B,
+                                     bk, rk, key, val);
         __syncthreads();
         // wave prefixes per bucket (in place) and sub-tile totals
         for (int b = threadIdx.x; b < B; b += kSplitBlock) {
@@ -141,7 +133,7 @@ __global__ void k_split_start(const int32_t* scanned, int64_t tiles, int B, int3
 // row, so no cross-wave work is needed.  The scatter starts each row at the
 // scanned column offsets, which turns the running counts into absolute
 // output positions.
-template <class K, int IPT>
+template <class K, int IPT, bool BALLOT>
 __global__ __launch_bounds__(kSplitBlock) void k_split_wave_hist(const K* __restrict__ kin, int64_t n, int lg,
                                                                  int shift, int64_t cols, int64_t colen,
                                                                  int32_t* __restrict__ his) {
@@ -154,17 +146,14 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_wave_hist(const K* __rest
     __builtin_amdgcn_wave_barrier();
     if (col >= cols) return;
     const int64_t c0 = col * colen, c1 = std::min<int64_t>(c0 + colen, n);
-    for (int64_t s = c0; s < c1; s += (int64_t)kWave * IPT) {
-        uint32_t bk[IPT], rk[IPT];
-        K key[IPT];
-        // rank_subtile addresses the calling wave's chunk at sbase + w*64*IPT
-        rank_subtile<K, IPT>(kin, s - (int64_t)w * kWave * IPT, c1, lg, shift, h, bk, rk, key);
-    }
+    // count_subtile addresses the calling wave's chunk at sbase + w*64*IPT
+    for (int64_t s = c0; s < c1; s += (int64_t)kWave * IPT)
+        count_subtile<K, IPT, BALLOT>(kin, s - (int64_t)w * kWave * IPT, c1, lg, shift, h);
     __builtin_amdgcn_wave_barrier();
     for (int b = lane_id(); b < B; b += kWave) his[(int64_t)b * cols + col] = (int32_t)h[b];
 }
 
-template <class K, int IPT, bool KV>
+template <class K, int IPT, bool KV, bool BALLOT>
 __global__ __launch_bounds__(kSplitBlock) void k_split_wave_scatter(const K* __restrict__ kin, K* __restrict__ kout,
                                                                     const int32_t* __restrict__ vin,
                                                                     int32_t* __restrict__ vout, int64_t n, int lg,
@@ -174,21 +163,27 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_wave_scatter(const K* __r
     const int B = 1 << lg;
     const int w = threadIdx.x / kWave;
     const int lane = lane_id();
-    uint32_t* h = lds + w * B;
+    uint64_t* wmask = reinterpret_cast<uint64_t*>(lds) + w * B;
+    uint32_t* h = lds + 2 * kSplitWaves * B + w * B;
     const int64_t col = (int64_t)blockIdx.x * kSplitWaves + w;
     if (col >= cols) return;
-    for (int b = lane; b < B; b += kWave) h[b] = (uint32_t)scanned[(int64_t)b * cols + col];
+    for (int b = lane; b < B; b += kWave) {
+        h[b] = (uint32_t)scanned[(int64_t)b * cols + col];
+        wmask[b] = 0ull;
+    }
     __builtin_amdgcn_wave_barrier();
     const int64_t c0 = col * colen, c1 = std::min<int64_t>(c0 + colen, n);
     for (int64_t s = c0; s < c1; s += (int64_t)kWave * IPT) {
         uint32_t bk[IPT], rk[IPT];
         K key[IPT];
-        rank_subtile<K, IPT>(kin, s - (int64_t)w * kWave * IPT, c1, lg, shift, h, bk, rk, key);
+        int32_t val[IPT];
+        rank_subtile<K, IPT, BALLOT>(kin, KV ? vin : nullptr, s - (int64_t)w * kWave * IPT, c1, lg, shift, h, wmask,
+                                     bk, rk, key, val);
         #pragma unroll
         for (int j = 0; j < IPT; ++j) {
             if (bk[j] != 0xFFFFFFFFu && rk[j] < (uint64_t)n) {  // rk is the output position
                 kout[rk[j]] = key[j];
-                if (KV) vout[rk[j]] = vin[s + (int64_t)j * kWave + lane];
+                if (KV) vout[rk[j]] = val[j];
             }
         }
     }
@@ -292,7 +287,7 @@ int64_t his_len(edpp_ctx* c, int strategy, int64_t n, int lg) {
     return B * single_lanes(c, n, lg);
 }
 
-template <class K, int IPT>
+template <class K, int IPT, bool BALLOT>
 int run_wg(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
     const int B = 1 << a.lg;
     constexpr int S = kSplitBlock * IPT;
@@ -306,13 +301,13 @@ int run_wg(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
     const int64_t hlen = (int64_t)B * g.tiles;
     size_t lh = (size_t)(kSplitWaves + 1) * B * 4;
     size_t ls = wg_lds_bytes<IPT, K>(B);
-    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_hist<K, IPT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_hist<K, IPT, BALLOT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lh));
-    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_scatter<K, IPT, true>,
+    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_scatter<K, IPT, true, BALLOT>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls));
-    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_scatter<K, IPT, false>,
+    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_scatter<K, IPT, false, BALLOT>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls));
-    hipLaunchKernelGGL((k_split_hist<K, IPT>), dim3((unsigned)g.tiles), dim3(kSplitBlock), lh, c->stream, a.kin, g,
+    hipLaunchKernelGGL((k_split_hist<K, IPT, BALLOT>), dim3((unsigned)g.tiles), dim3(kSplitBlock), lh, c->stream, a.kin, g,
                        his);
     EDPP_HIP(c, hipGetLastError());
     int rc = launch_scan<uint32_t>(c, (const uint32_t*)his, (uint32_t*)his, hlen, EDPP_SCAN_LOOKBACK, nullptr);
@@ -321,30 +316,31 @@ int run_wg(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
         hipLaunchKernelGGL(k_split_start, dim3((unsigned)ceil_div(B, 256)), dim3(256), 0, c->stream,
                            (const int32_t*)his, g.tiles, B, a.d_start);
     if (a.vin)
-        hipLaunchKernelGGL((k_split_scatter<K, IPT, true>), dim3((unsigned)g.tiles), dim3(kSplitBlock), ls, c->stream,
+        hipLaunchKernelGGL((k_split_scatter<K, IPT, true, BALLOT>), dim3((unsigned)g.tiles), dim3(kSplitBlock), ls, c->stream,
                            a.kin, a.kout, a.vin, a.vout, g, (const int32_t*)his);
     else
-        hipLaunchKernelGGL((k_split_scatter<K, IPT, false>), dim3((unsigned)g.tiles), dim3(kSplitBlock), ls,
+        hipLaunchKernelGGL((k_split_scatter<K, IPT, false, BALLOT>), dim3((unsigned)g.tiles), dim3(kSplitBlock), ls,
                            c->stream, a.kin, a.kout, a.vin, a.vout, g, (const int32_t*)his);
     EDPP_HIP(c, hipGetLastError());
     return EDPP_OK;
 }
 
-template <class K, int IPT>
+template <class K, int IPT, bool BALLOT>
 int run_wi(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
     const int B = 1 << a.lg;
     const int64_t colen = wi_colen(c, a.lg, IPT);
     const int64_t cols = ceil_div(a.n, colen);
     const int64_t hlen = (int64_t)B * cols;
     const unsigned grid = (unsigned)ceil_div(cols, kSplitWaves);
-    const size_t lds = (size_t)kSplitWaves * B * 4;
-    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_wave_hist<K, IPT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    const size_t lds = (size_t)kSplitWaves * B * 4;        // histogram kernel: counts
+    const size_t lds_s = (size_t)kSplitWaves * B * 12;     // scatter kernel: masks and positions
+    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_wave_hist<K, IPT, BALLOT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds));
-    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_wave_scatter<K, IPT, true>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_wave_scatter<K, IPT, false>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_split_wave_hist<K, IPT>), dim3(grid), dim3(kSplitBlock), lds, c->stream, a.kin, a.n, a.lg,
+    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_wave_scatter<K, IPT, true, BALLOT>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_s));
+    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_wave_scatter<K, IPT, false, BALLOT>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_s));
+    hipLaunchKernelGGL((k_split_wave_hist<K, IPT, BALLOT>), dim3(grid), dim3(kSplitBlock), lds, c->stream, a.kin, a.n, a.lg,
                        a.shift, cols, colen, his);
     EDPP_HIP(c, hipGetLastError());
     int rc = launch_scan<uint32_t>(c, (const uint32_t*)his, (uint32_t*)his, hlen, EDPP_SCAN_LOOKBACK, nullptr);
@@ -353,10 +349,10 @@ int run_wi(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
         hipLaunchKernelGGL(k_split_start, dim3((unsigned)ceil_div(B, 256)), dim3(256), 0, c->stream,
                            (const int32_t*)his, cols, B, a.d_start);
     if (a.vin)
-        hipLaunchKernelGGL((k_split_wave_scatter<K, IPT, true>), dim3(grid), dim3(kSplitBlock), lds, c->stream, a.kin,
+        hipLaunchKernelGGL((k_split_wave_scatter<K, IPT, true, BALLOT>), dim3(grid), dim3(kSplitBlock), lds_s, c->stream, a.kin,
                            a.kout, a.vin, a.vout, a.n, a.lg, a.shift, cols, colen, (const int32_t*)his);
     else
-        hipLaunchKernelGGL((k_split_wave_scatter<K, IPT, false>), dim3(grid), dim3(kSplitBlock), lds, c->stream,
+        hipLaunchKernelGGL((k_split_wave_scatter<K, IPT, false, BALLOT>), dim3(grid), dim3(kSplitBlock), lds_s, c->stream,
                            a.kin, a.kout, a.vin, a.vout, a.n, a.lg, a.shift, cols, colen, (const int32_t*)his);
     EDPP_HIP(c, hipGetLastError());
     return EDPP_OK;
@@ -390,10 +386,14 @@ int run_single(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
 template <class K>
 int run_pass(edpp_ctx* c, int strategy, const SplitCall<K>& a, int32_t* his) {
     switch (strategy) {
-        case EDPP_SPLIT_WG:
-            if (wg_ipt<K>(c, a.lg) == 16) return run_wg<K, 16>(c, a, his);
-            return run_wg<K, 8>(c, a, his);
-        case EDPP_SPLIT_WI: return run_wi<K, 16>(c, a, his);
+        case EDPP_SPLIT_WG: {
+            const bool bal = a.lg <= kBallotMatchBits;
+            if (wg_ipt<K>(c, a.lg) == 16)
+                return bal ? run_wg<K, 16, true>(c, a, his) : run_wg<K, 16, false>(c, a, his);
+            return bal ? run_wg<K, 8, true>(c, a, his) : run_wg<K, 8, false>(c, a, his);
+        }
+        case EDPP_SPLIT_WI:
+            return a.lg <= kBallotMatchBits ? run_wi<K, 16, true>(c, a, his) : run_wi<K, 16, false>(c, a, his);
         default: return run_single<K>(c, a, his);
     }
 }

@@ -9,7 +9,8 @@ namespace edpp {
 constexpr int kSplitBlock = 256;
 constexpr int kSplitWaves = kSplitBlock / kWave;
 
-// Mask of the lanes whose bucket equals the caller's, among valid lanes.
+// Mask of the lanes whose bucket equals the caller's, among valid lanes: one
+// ballot per bucket bit (cheap while B is small).
 __device__ __forceinline__ uint64_t match_bucket(uint32_t b, int lg, uint64_t valid) {
     uint64_t m = valid;
     for (int bit = 0; bit < lg; ++bit) {
@@ -19,6 +20,10 @@ __device__ __forceinline__ uint64_t match_bucket(uint32_t b, int lg, uint64_t va
     }
     return m;
 }
+
+// Up to this many bucket bits the ballot match is cheaper than LDS matching,
+// whose same-address atomics serialize when many lanes share few buckets.
+constexpr int kBallotMatchBits = 4;
 
 // XCD-aware tile order: workgroups are dispatched round-robin over the XCDs,
 // so workgroup g runs on XCD g % xcds.  Give every XCD a contiguous range of
@@ -33,10 +38,12 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t g, int64_t tiles, int xcds) 
     int64_t start = x * per + (x < rem ? x : rem);
     return start + k;
 }
-// LDS layout of the WG scatter: whist[waves][B], run[B], sub[B], off[B],
-// wtmp[16], then the key and value reorder buffers from a 16-byte boundary
+// LDS layout of the WG scatter (32-bit words): wmask[waves][B] (64-bit),
+// whist[waves][B], run[B], sub[B], off[B], wtmp[16], then the key and value
+// reorder buffers from a 16-byte boundary
+__host__ __device__ __forceinline__ int whist_word_offset(int B) { return 2 * kSplitWaves * B; }
 __host__ __device__ __forceinline__ int kbuf_word_offset(int B) {
-    return (((kSplitWaves + 3) * B + 16) + 3) & ~3;
+    return ((whist_word_offset(B) + (kSplitWaves + 3) * B + 16) + 3) & ~3;
 }
 
 // ------------------------------------------------------------- WG strategy
@@ -53,34 +60,95 @@ struct SplitGeom {
     int xcds;
 };
 
-// per-wave counts of one sub-tile into whist[w][b] (zeroed by the caller)
+// Loads the calling wave's chunk of a sub-tile (item j of lane l is element
+// sbase + w*64*IPT + j*64 + l), and its values when vin is given, then
+// extracts buckets (0xFFFFFFFF past n).  Every load is issued before any is
+// used (the scheduling barrier keeps the compiler from consuming, and so
+// waiting for, each load before issuing the next).
 template <class K, int IPT>
-__device__ __forceinline__ void rank_subtile(const K* __restrict__ kin, int64_t sbase, int64_t n, int lg, int shift,
-                                             uint32_t* whist_w, uint32_t (&bk)[IPT], uint32_t (&rk)[IPT],
-                                             K (&key)[IPT]) {
+__device__ __forceinline__ void load_subtile(const K* __restrict__ kin, const int32_t* __restrict__ vin, int64_t sbase,
+                                             int64_t n, int lg, int shift, uint32_t (&bk)[IPT], K (&key)[IPT],
+                                             int32_t (&val)[IPT]) {
     const int lane = lane_id();
     const int w = threadIdx.x / kWave;
     const uint32_t mask = (1u << lg) - 1u;
     const int64_t wbase = sbase + (int64_t)w * kWave * IPT;
     #pragma unroll
     for (int j = 0; j < IPT; ++j) {
-        int64_t e = wbase + (int64_t)j * kWave + lane;
-        bool ok = e < n;
-        key[j] = ok ? kin[e] : K(0);
-        bk[j] = ok ? (((uint32_t)key[j] >> shift) & mask) : 0xFFFFFFFFu;
+        const int64_t e = wbase + (int64_t)j * kWave + lane;
+        key[j] = e < n ? kin[e] : K(0);
     }
+    if (vin) {
+        #pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            const int64_t e = wbase + (int64_t)j * kWave + lane;
+            val[j] = e < n ? vin[e] : 0;
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
     #pragma unroll
     for (int j = 0; j < IPT; ++j) {
-        bool ok = bk[j] != 0xFFFFFFFFu;
-        uint64_t valid = ballot(ok);
-        uint64_t m = match_bucket(bk[j], lg, valid);
-        uint32_t below = count_below(m);
-        uint32_t cnt = (uint32_t)__popcll(m);
-        uint32_t prior = ok ? whist_w[bk[j]] : 0u;
+        const int64_t e = wbase + (int64_t)j * kWave + lane;
+        bk[j] = e < n ? (((uint32_t)key[j] >> shift) & mask) : 0xFFFFFFFFu;
+    }
+}
+
+// Bucket counts of the wave's chunk added to whist_w[b] (order-free).
+// BALLOT: one lane per group adds the group's size (few buckets, where LDS
+// atomics would collide); otherwise plain LDS atomics.
+template <class K, int IPT, bool BALLOT>
+__device__ __forceinline__ void count_subtile(const K* __restrict__ kin, int64_t sbase, int64_t n, int lg, int shift,
+                                              uint32_t* whist_w) {
+    uint32_t bk[IPT];
+    K key[IPT];
+    int32_t val[IPT];
+    load_subtile<K, IPT>(kin, nullptr, sbase, n, lg, shift, bk, key, val);
+    #pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        const bool ok = bk[j] != 0xFFFFFFFFu;
+        if (BALLOT) {
+            const uint64_t m = match_bucket(bk[j], lg, ballot(ok));
+            if (ok && count_below(m) == 0) whist_w[bk[j]] += (uint32_t)__popcll(m);
+        } else if (ok) {
+            atomicAdd(&whist_w[bk[j]], 1u);
+        }
+    }
+}
+
+// Stable ranks of the wave's chunk: rk[j] = whist_w[b] (count of earlier keys
+// of bucket b) + number of lanes below with the same bucket in item j; the
+// last lane of each group advances whist_w[b].  The lanes sharing a bucket
+// come from bucket-bit ballots (BALLOT) or from wmask[b] (per wave, zero on
+// entry and on exit): every lane ORs in its own bit, reads the mask back and
+// clears it.  OR is order-independent and a wave's LDS instructions complete
+// in program order (each lane's accesses to its own slot are ordered by the
+// same address), so the mask is exact.
+template <class K, int IPT, bool BALLOT>
+__device__ __forceinline__ void rank_subtile(const K* __restrict__ kin, const int32_t* __restrict__ vin,
+                                             int64_t sbase, int64_t n, int lg, int shift, uint32_t* whist_w,
+                                             uint64_t* wmask, uint32_t (&bk)[IPT], uint32_t (&rk)[IPT],
+                                             K (&key)[IPT], int32_t (&val)[IPT]) {
+    const int lane = lane_id();
+    load_subtile<K, IPT>(kin, vin, sbase, n, lg, shift, bk, key, val);
+    #pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        const bool ok = bk[j] != 0xFFFFFFFFu;
+        const uint32_t b = ok ? bk[j] : 0u;
+        uint64_t m;
+        if (BALLOT) {
+            m = match_bucket(bk[j], lg, ballot(ok));
+        } else {
+            if (ok) __hip_atomic_fetch_or(wmask + b, 1ull << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            m = ok ? wmask[b] : 0ull;
+        }
+        const uint32_t prior = ok ? whist_w[b] : 0u;
+        const uint32_t below = count_below(m);
+        const uint32_t cnt = (uint32_t)__popcll(m);
         rk[j] = prior + below;
-        // the highest lane of each group updates the count; the read above
-        // precedes this write in the wave's LDS instruction order
-        if (ok && below == cnt - 1) whist_w[bk[j]] = prior + cnt;
+        if (ok) {
+            if (!BALLOT) wmask[b] = 0ull;
+            if (below == cnt - 1) whist_w[b] = prior + cnt;
+        }
     }
 }
 
