@@ -11,8 +11,10 @@ scatter over pass counts and index kinds, stable split over bucket counts and
 key skews) under the key "suite" of the same JSON line.
 
 Timing: HIP events recorded on the library context's stream, which is torch's
-current stream, around exactly K steps after W warm-up steps; per-kernel
-averages come from events around the individual calls.
+current stream, around exactly K steps after W warm-up steps; the dominant
+kernel's launch time comes from events around its calls inside those steps.
+``roofline.traffic`` (HBM bytes per launch) comes from two rocprofv3 --pmc
+child runs (FETCH_SIZE, WRITE_SIZE) made before the parent touches the GPU.
 """
 import argparse
 import json
@@ -43,6 +45,8 @@ def parse():
     p.add_argument("--test-calls", type=int, default=500)
     p.add_argument("--db-out", default=None, help="write the autotuner database (JSON lines) here")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--no-traffic", action="store_true",
+                   help="skip the two rocprofv3 --pmc child runs that measure HBM bytes per launch")
     p.add_argument("--cpu-sample", type=int, default=1 << 26, help="elements in the CPU baseline sample")
     p.add_argument("--passes", type=int, default=0, help="pass count of the default step (0 = library schedule)")
     p.add_argument("--out", default=None, help="also write the JSON line to this file")
@@ -132,6 +136,46 @@ def cpu_baseline(args, torch_unused=None):
     }
 
 
+def pmc_traffic(kernel_prefix, profile, passes, timeout_s=150):
+    """HBM bytes per launch of one kernel from two rocprofv3 --pmc child runs
+    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) of ``bench.py --profile``.
+    Each counter is scaled by what it reads for the STREAM copy kernel of 2^28
+    int32 in the same run (1 GiB read, 1 GiB written).  Returns a dict, or
+    None with the reason when the profiler is absent or fails."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3") or ("/opt/rocm/bin/rocprofv3" if os.path.exists("/opt/rocm/bin/rocprofv3")
+                                         else None)
+    if prof is None:
+        return None, "rocprofv3 not found"
+    known = float(N_FULL * 4) / 1024.0  # KiB read (FETCH) or written (WRITE) by the copy
+    out = {}
+    env = dict(os.environ, TMPDIR="/tmp")
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="edpp_pmc_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", str(timeout_s), prof, "--pmc", ctr, "--output-format", "csv", "-d", d,
+               "-o", "run", "--", sys.executable, os.path.join(REPO, "bench.py"), "--profile", profile,
+               "--steps", "2", "--passes", str(passes)]
+        r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True)
+        rows = []
+        for root, _, files in os.walk(d):
+            for fn in files:
+                if fn.endswith("counter_collection.csv"):
+                    with open(os.path.join(root, fn), newline="") as f:
+                        rows += [row for row in csv.DictReader(f) if row.get("Counter_Name") == ctr]
+        shutil.rmtree(d, ignore_errors=True)
+        kern = [float(rw["Counter_Value"]) for rw in rows if rw["Kernel_Name"].startswith(kernel_prefix)]
+        copy = [float(rw["Counter_Value"]) for rw in rows if rw["Kernel_Name"].startswith("void edpp::k_stream<0,")]
+        if r.returncode != 0 or not kern or not copy:
+            return None, f"{ctr} run rc={r.returncode}, {len(kern)} kernel / {len(copy)} copy dispatches"
+        factor = known / float(np.mean(copy))
+        out[ctr] = {"kib_per_dispatch": float(np.mean(kern)), "copy_kib": float(np.mean(copy)),
+                    "factor": round(factor, 4), "bytes": float(np.mean(kern)) * factor * 1024.0}
+    return out, None
+
+
 def profile_only(args, ctx, edpp, torch):
     """One workload at full size, `steps` times, then one STREAM copy of 2^28
     int32 (the calibration kernel for the byte counters)."""
@@ -185,6 +229,11 @@ def main():
         profile_only(args, ctx, edpp, torch)
         ctx.close()
         return
+    # HBM bytes per launch of the dominant kernel, from profiler child runs made
+    # before this process touches the GPU
+    traffic, traffic_note = (None, "skipped (--no-traffic)")
+    if not args.no_traffic:
+        traffic, traffic_note = pmc_traffic("void edpp::k_gather<", "gather", args.passes)
     import importlib
     import torch
     edpp = importlib.import_module("efficient-dpp_amd")
@@ -203,17 +252,27 @@ def main():
     dst2 = torch.empty(n, dtype=i32, device="cuda")
     passes = args.passes
 
-    def step():
+    # events on the context's stream: around each step and around each primitive
+    # inside it, so the dominant kernel's launch time comes from the timed region
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+
+    def step(k=None):
+        if k is not None:
+            ev[k][0].record()
         ctx.gather(src, dst, idx, n, passes)
+        if k is not None:
+            ev[k][1].record()
         ctx.scatter(dst, dst2, idx, n, passes)
+        if k is not None:
+            ev[k][2].record()
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
-    for _ in range(args.steps):
-        step()
+    for k in range(args.steps):
+        step(k)
     e1.record()
     torch.cuda.synchronize()
     total_ms = e0.elapsed_time(e1)
@@ -223,11 +282,12 @@ def main():
     # bit-exact spot check of the step's result (scatter undoes gather)
     assert torch.equal(dst2, src), "gather/scatter round trip mismatch"
 
-    # dominant kernel: the gather launch(es); per-launch average from events
-    g = T.time(lambda: ctx.gather(src, dst, idx, n, passes), reps=max(5, args.steps))
-    sc = T.time(lambda: ctx.scatter(dst, dst2, idx, n, passes), reps=max(5, args.steps))
-    eff_passes = passes if passes > 0 else 1
-    achieved = gbs(12.0 * n, g["hampel_ms"])
+    # dominant kernel: the gather launch(es) of the timed steps
+    g_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+    s_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+    g_passes = passes if passes > 0 else ctx.schedule_passes(n, scatter=False)
+    s_passes = passes if passes > 0 else ctx.schedule_passes(n, scatter=True)
+    achieved = gbs(12.0 * n, g_ms)
     copy = T.time(lambda: ctx.stream_op(edpp.STREAM_COPY, src, None, dst, n), reps=10)
     copy_gbs = gbs(8.0 * n, copy["hampel_ms"])
 
@@ -245,14 +305,20 @@ def main():
         "dtype": "int32",
         "data": "synthetic (SplitMix64 values, keyed-Feistel random permutation; generated on device)",
         "config": {"workload": "multi-pass gather + scatter, 2^28 int32, full random permutation",
-                   "n": n, "passes": eff_passes if passes else "schedule", "index": "random permutation",
+                   "n": n, "gather_passes": g_passes, "scatter_passes": s_passes,
+                   "passes_from": "argument" if passes else "library schedule", "index": "random permutation",
                    "bytes_per_element_per_primitive": 12},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(traffic["FETCH_SIZE"]["bytes"] + traffic["WRITE_SIZE"]["bytes"]) if traffic
+                     else None,
                      "kernel": "k_gather (one launch per pass)",
-                     "avg_launch_ms": round(g["hampel_ms"] / eff_passes, 4),
-                     "vs_stream_copy": round(achieved / copy_gbs, 4), "stream_copy_gbs": round(copy_gbs, 2)},
-        "per_primitive": {"gather_gbs": round(achieved, 2), "scatter_gbs": round(gbs(12.0 * n, sc["hampel_ms"]), 2)},
+                     "algorithmic_bytes_per_launch": 12 * n // g_passes,
+                     "avg_launch_ms": round(g_ms / g_passes, 4),
+                     "vs_stream_copy": round(achieved / copy_gbs, 4), "stream_copy_gbs": round(copy_gbs, 2),
+                     "traffic_counters": traffic if traffic else traffic_note},
+        "per_primitive": {"gather_gbs": round(achieved, 2), "scatter_gbs": round(gbs(12.0 * n, s_ms), 2),
+                          "gather_ms": round(g_ms, 4), "scatter_ms": round(s_ms, 4)},
     }
     del dst2
     if args.suite:

@@ -44,6 +44,7 @@ _SIGS = {
     "edpp_scatter_i32": [c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_dp],
     "edpp_gather_i64": [c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_dp],
     "edpp_scatter_i64": [c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_dp],
+    "edpp_default_passes": [c_vp, c_i64, c_int, c_int, ctypes.POINTER(c_int)],
     "edpp_scan_exclusive_i32": [c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_dp],
     "edpp_scan_exclusive_i64": [c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_dp],
     "edpp_scan_local_i32": [c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_dp],
@@ -255,6 +256,13 @@ class Context:
         n = _numel(idx) if n is None else n
         name = "edpp_scatter_i64" if wide else "edpp_scatter_i32"
         return self._call(name, _ptr(src), _ptr(dst), _ptr(idx), n, int(passes), timed=timed)
+
+    def schedule_passes(self, n, scatter=False, wide=False):
+        """Pass count a passes=0 gather/scatter of n elements uses without a cost model."""
+        out = c_int(0)
+        self._check(self._L.edpp_default_passes(self.h, int(n), 8 if wide else 4, int(bool(scatter)),
+                                                ctypes.byref(out)))
+        return out.value
 
     def scan(self, src, dst, n=None, scheme=SCAN_AUTO, total=None, timed=False, wide=False):
         n = _numel(src) if n is None else n

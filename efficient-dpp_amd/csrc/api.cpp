@@ -344,6 +344,12 @@ int edpp_scatter_i64(edpp_ctx* c, const int64_t* in, int64_t* out, const int32_t
                      double* ms) {
     return gs_common<uint64_t>(c, (const uint64_t*)in, (uint64_t*)out, idx, n, passes, ms, true);
 }
+int edpp_default_passes(edpp_ctx* c, int64_t n, int elem_bytes, int scatter, int* passes) {
+    EDPP_CHECK_CTX(c);
+    if (!passes || n < 0 || (elem_bytes != 4 && elem_bytes != 8)) return fail(c, EDPP_EINVAL, "edpp_default_passes: bad argument");
+    *passes = default_passes(c, n, elem_bytes, scatter != 0);
+    return EDPP_OK;
+}
 
 // ---------------------------------------------------------------- scans
 }  // extern "C"

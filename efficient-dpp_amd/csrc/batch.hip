@@ -23,6 +23,7 @@ namespace edpp {
 namespace {
 
 constexpr int kBatchBlock = 256;
+static_assert(kBatchBlock == kSplitBlock, "split tiles assume the split block size");
 constexpr int kPhaseHist = 0, kPhaseHistScan = 1, kPhaseMain = 2, kPhaseScatter = 3;
 
 struct DJob {
@@ -144,9 +145,7 @@ __device__ void split_scatter_tile(const DJob& J, int64_t t, uint32_t* lds) {
     uint32_t* wtmp = off + B;
     K* kbuf = reinterpret_cast<K*>(lds + kbuf_word_offset(B));
     int32_t* vbuf = reinterpret_cast<int32_t*>(kbuf + S);
-    const uint32_t mask = (uint32_t)B - 1u;
     const int w = threadIdx.x / kWave;
-    const int lane = lane_id();
     const bool kv = J.vin != nullptr;
     for (int b = threadIdx.x; b < B; b += kBatchBlock) {
         uint32_t r = (uint32_t)ld_relaxed(J.his + (int64_t)b * J.tiles + t);
@@ -169,40 +168,13 @@ __device__ void split_scatter_tile(const DJob& J, int64_t t, uint32_t* lds) {
         rank_subtile<K, IPT, false>((const K*)J.in, J.vin, s, tend, J.lg, 0, whist + w * B, wmask + w * B, bk, rk,
                                     key, val);
     __syncthreads();
-    for (int b = threadIdx.x; b < B; b += kBatchBlock) {
-        uint32_t a = 0;
-        for (int q = 0; q < kSplitWaves; ++q) {
-            uint32_t x = whist[q * B + b];
-            whist[q * B + b] = a;
-            a += x;
-        }
-        sub[b] = a;
-    }
-    __syncthreads();
-    scan_buckets(sub, off, B, wtmp);
-    __syncthreads();
-    #pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-        if (bk[j] != 0xFFFFFFFFu) {
-            uint32_t p = off[bk[j]] + whist[w * B + bk[j]] + rk[j];
-            if (p < (uint32_t)S) {
-                kbuf[p] = key[j];
-                if (kv) vbuf[p] = val[j];
-            }
-        }
-    }
-    __syncthreads();
     const int cnt = (int)(tend - s);
-    K* kout = static_cast<K*>(J.out);
-    for (int i = threadIdx.x; i < cnt; i += kBatchBlock) {
-        K k = kbuf[i];
-        uint32_t b = (uint32_t)k & mask;
-        uint32_t pos = run[b] + ((uint32_t)i - off[b]);
-        if (pos < (uint64_t)J.n) {
-            kout[pos] = k;
-            if (kv) J.vout[pos] = vbuf[i];
-        }
-    }
+    if (kv)
+        place_and_store<K, IPT, true>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk, key, val, B, 0, cnt,
+                                      static_cast<K*>(J.out), J.vout, J.n);
+    else
+        place_and_store<K, IPT, false>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk, key, val, B, 0, cnt,
+                                       static_cast<K*>(J.out), nullptr, J.n);
 }
 
 // ---------------------------------------------------------------- gather tiles

@@ -61,12 +61,10 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_scatter(const K* __restri
     uint32_t* wtmp = off + B;                    // [waves]
     K* kbuf = reinterpret_cast<K*>(lds + kbuf_word_offset(B));  // [S], 16-byte aligned
     int32_t* vbuf = reinterpret_cast<int32_t*>(kbuf + S);
-    const uint32_t mask = (uint32_t)B - 1u;
     const int64_t tile = xcd_tile(blockIdx.x, g.tiles, g.xcds);
     const int64_t tbase = tile * g.tile;
     const int64_t tend = std::min<int64_t>(tbase + g.tile, g.n);
     const int w = threadIdx.x / kWave;
-    const int lane = lane_id();
     for (int b = threadIdx.x; b < B; b += kSplitBlock) run[b] = (uint32_t)scanned[(int64_t)b * g.tiles + tile];
     for (int b = threadIdx.x; b < kSplitWaves * B; b += kSplitBlock) wmask[b] = 0ull;
     for (int64_t s = tbase; s < tend; s += S) {
@@ -78,46 +76,10 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_scatter(const K* __restri
         rank_subtile<K, IPT, BALLOT>(kin, KV ? vin : nullptr, s, tend, g.lg, g.shift, whist + w * B, wmask + w * B,
                                      bk, rk, key, val);
         __syncthreads();
-        // wave prefixes per bucket (in place) and sub-tile totals
-        for (int b = threadIdx.x; b < B; b += kSplitBlock) {
-            uint32_t a = 0;
-            for (int q = 0; q < kSplitWaves; ++q) {
-                uint32_t x = whist[q * B + b];
-                whist[q * B + b] = a;
-                a += x;
-            }
-            sub[b] = a;
-        }
-        __syncthreads();
-        scan_buckets(sub, off, B, wtmp);
-        __syncthreads();
-        // place every element at its stable position inside the sub-tile
-        #pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-            if (bk[j] != 0xFFFFFFFFu) {
-                uint32_t p = off[bk[j]] + whist[w * B + bk[j]] + rk[j];
-                if (p < (uint32_t)S) {  // always true for consistent ranks
-                    kbuf[p] = key[j];
-                    if (KV) vbuf[p] = val[j];
-                }
-            }
-        }
-        __syncthreads();
-        // stream the sub-tile out: consecutive slots of one bucket go to
-        // consecutive output positions
-        const int cnt = (int)std::min<int64_t>((int64_t)S, tend - s);
-        for (int i = threadIdx.x; i < cnt; i += kSplitBlock) {
-            K k = kbuf[i];
-            uint32_t b = ((uint32_t)k >> g.shift) & mask;
-            uint32_t pos = run[b] + ((uint32_t)i - off[b]);
-            if (pos < (uint64_t)g.n) {  // always true for a consistent histogram scan
-                kout[pos] = k;
-                if (KV) vout[pos] = vbuf[i];
-            }
-        }
-        __syncthreads();
-        for (int b = threadIdx.x; b < B; b += kSplitBlock) run[b] += sub[b];
-        // the next iteration's first barrier orders these updates before use
+        place_and_store<K, IPT, KV>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk, key, val, B, g.shift,
+                                    (int)std::min<int64_t>((int64_t)S, tend - s), kout, vout, g.n);
+        // the next iteration's first barrier ends every thread's stores
+        // before kbuf, sub and run are used again
     }
 }
 

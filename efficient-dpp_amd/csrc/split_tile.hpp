@@ -55,7 +55,7 @@ struct SplitGeom {
     int64_t n;
     int lg;          // log2 B
     int shift;       // bucket = (key >> shift) & (B-1)
-    int64_t tile;    // elements per tile
+    int64_t tile;    // elements per tile (a multiple of the sub-tile)
     int64_t tiles;
     int xcds;
 };
@@ -108,21 +108,26 @@ __device__ __forceinline__ void count_subtile(const K* __restrict__ kin, int64_t
         const bool ok = bk[j] != 0xFFFFFFFFu;
         if (BALLOT) {
             const uint64_t m = match_bucket(bk[j], lg, ballot(ok));
-            if (ok && count_below(m) == 0) whist_w[bk[j]] += (uint32_t)__popcll(m);
+            if (ok && count_below(m) == 0)
+                __hip_atomic_fetch_add(whist_w + bk[j], (uint32_t)__popcll(m), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
         } else if (ok) {
-            atomicAdd(&whist_w[bk[j]], 1u);
+            __hip_atomic_fetch_add(whist_w + bk[j], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         }
     }
 }
 
 // Stable ranks of the wave's chunk: rk[j] = whist_w[b] (count of earlier keys
-// of bucket b) + number of lanes below with the same bucket in item j; the
-// last lane of each group advances whist_w[b].  The lanes sharing a bucket
-// come from bucket-bit ballots (BALLOT) or from wmask[b] (per wave, zero on
+// of bucket b) + number of lanes below with the same bucket in item j.
+//
+// Phase 1 finds, for every item, the mask of lanes sharing the caller's
+// bucket: from bucket-bit ballots (BALLOT) or from wmask[b] (per wave, zero on
 // entry and on exit): every lane ORs in its own bit, reads the mask back and
-// clears it.  OR is order-independent and a wave's LDS instructions complete
-// in program order (each lane's accesses to its own slot are ordered by the
-// same address), so the mask is exact.
+// clears it.  Phase 2 reads whist_w[b] and lets the lowest lane of each group
+// add the group size.  A wave's LDS instructions execute in program order, so
+// neither phase waits for an earlier item's result before issuing the next
+// item: item j's mask read sees exactly item j's ORs (item j-1 cleared its
+// slots before them), and item j's whist_w read sees every add of items < j.
 template <class K, int IPT, bool BALLOT>
 __device__ __forceinline__ void rank_subtile(const K* __restrict__ kin, const int32_t* __restrict__ vin,
                                              int64_t sbase, int64_t n, int lg, int shift, uint32_t* whist_w,
@@ -130,25 +135,30 @@ __device__ __forceinline__ void rank_subtile(const K* __restrict__ kin, const in
                                              K (&key)[IPT], int32_t (&val)[IPT]) {
     const int lane = lane_id();
     load_subtile<K, IPT>(kin, vin, sbase, n, lg, shift, bk, key, val);
+    uint64_t m[IPT];
+    #pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        const bool ok = bk[j] != 0xFFFFFFFFu;
+        if (BALLOT) {
+            m[j] = match_bucket(bk[j], lg, ballot(ok));
+        } else {
+            const uint32_t b = ok ? bk[j] : 0u;
+            if (ok) __hip_atomic_fetch_or(wmask + b, 1ull << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            m[j] = ok ? __hip_atomic_load(wmask + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) : 0ull;
+            if (ok) __hip_atomic_store(wmask + b, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        }
+    }
     #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         const bool ok = bk[j] != 0xFFFFFFFFu;
         const uint32_t b = ok ? bk[j] : 0u;
-        uint64_t m;
-        if (BALLOT) {
-            m = match_bucket(bk[j], lg, ballot(ok));
-        } else {
-            if (ok) __hip_atomic_fetch_or(wmask + b, 1ull << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-            m = ok ? wmask[b] : 0ull;
-        }
-        const uint32_t prior = ok ? whist_w[b] : 0u;
-        const uint32_t below = count_below(m);
-        const uint32_t cnt = (uint32_t)__popcll(m);
+        const uint32_t prior =
+            ok ? __hip_atomic_load(whist_w + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) : 0u;
+        const uint32_t below = count_below(m[j]);
+        if (ok && below == 0)
+            __hip_atomic_fetch_add(whist_w + b, (uint32_t)__popcll(m[j]), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
         rk[j] = prior + below;
-        if (ok) {
-            if (!BALLOT) wmask[b] = 0ull;
-            if (below == cnt - 1) whist_w[b] = prior + cnt;
-        }
     }
 }
 
@@ -173,6 +183,71 @@ __device__ __forceinline__ void scan_buckets(const uint32_t* cnt, uint32_t* off,
             off[b0 + q] = x;
             x += c;
         }
+}
+
+// Second half of a WG-strategy sub-tile, after rank_subtile: per-wave counts
+// become sub-tile offsets, the sub-tile is reordered by bucket in LDS (kbuf,
+// vbuf), and slot i of bucket b is written to run[b] + (i - off[b]), so the
+// consecutive slots of a bucket leave as consecutive stores.  run[b] (each
+// bucket's next output position) advances by the sub-tile's count of b; sub[]
+// is reused for run[b] - off[b].  cnt is the number of valid elements (S for
+// every sub-tile but the last).  The stream-out is a plain loop: each store
+// leaves as soon as its slot is read, which measured faster than reading
+// every slot first and storing in one burst.  Block of kSplitBlock threads;
+// ends without a barrier.
+template <class K, int IPT, bool KV>
+__device__ __forceinline__ void place_and_store(uint32_t* whist, uint32_t* run, uint32_t* sub, uint32_t* off,
+                                                uint32_t* wtmp, K* kbuf, int32_t* vbuf, const uint32_t (&bk)[IPT],
+                                                const uint32_t (&rk)[IPT], const K (&key)[IPT],
+                                                const int32_t (&val)[IPT], int B, int shift, int cnt,
+                                                K* __restrict__ kout, int32_t* __restrict__ vout, int64_t n) {
+    constexpr int S = kSplitBlock * IPT;
+    const int w = threadIdx.x / kWave;
+    const uint32_t mask = (uint32_t)B - 1u;
+    // wave prefixes per bucket (in place) and sub-tile totals
+    for (int b = threadIdx.x; b < B; b += kSplitBlock) {
+        uint32_t x[kSplitWaves];
+        #pragma unroll
+        for (int q = 0; q < kSplitWaves; ++q) x[q] = whist[q * B + b];
+        uint32_t a = 0;
+        #pragma unroll
+        for (int q = 0; q < kSplitWaves; ++q) {
+            whist[q * B + b] = a;
+            a += x[q];
+        }
+        sub[b] = a;
+    }
+    __syncthreads();
+    scan_buckets(sub, off, B, wtmp);
+    __syncthreads();
+    uint32_t pos[IPT];
+    #pragma unroll
+    for (int j = 0; j < IPT; ++j)
+        pos[j] = bk[j] != 0xFFFFFFFFu ? off[bk[j]] + whist[w * B + bk[j]] + rk[j] : (uint32_t)S;
+    #pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        if (pos[j] < (uint32_t)S) {  // false only past n, for consistent ranks
+            kbuf[pos[j]] = key[j];
+            if (KV) vbuf[pos[j]] = val[j];
+        }
+    }
+    // each thread owns its buckets here: nobody reads sub[] or run[] again
+    // before the barrier below
+    for (int b = threadIdx.x; b < B; b += kSplitBlock) {
+        const uint32_t c = sub[b];
+        sub[b] = run[b] - off[b];
+        run[b] += c;
+    }
+    __syncthreads();
+    const uint32_t* dlt = sub;
+    for (int i = threadIdx.x; i < cnt; i += kSplitBlock) {
+        const K k = kbuf[i];
+        const uint32_t p = dlt[((uint32_t)k >> shift) & mask] + (uint32_t)i;
+        if ((int64_t)p < n) {  // always true for a consistent histogram scan
+            kout[p] = k;
+            if (KV) vout[p] = vbuf[i];
+        }
+    }
 }
 
 }  // namespace edpp

@@ -136,6 +136,10 @@ int edpp_gather_i64 (edpp_ctx* ctx, const int64_t* in, int64_t* out, const int32
                      int64_t n, int passes, double* ms);
 int edpp_scatter_i64(edpp_ctx* ctx, const int64_t* in, int64_t* out, const int32_t* idx,
                      int64_t n, int passes, double* ms);
+/* The pass count a passes == 0 call on n elements of elem_bytes (4 or 8) uses
+ * while no cost model is set (the built-in schedule).  No reference
+ * counterpart: the reference always takes the pass count from its caller.   */
+int edpp_default_passes(edpp_ctx* ctx, int64_t n, int elem_bytes, int scatter, int* passes);
 
 /* ---- global exclusive scan ----------------------------------------------
  * opencl/primitives/scanImpl.cpp:15-236.  in == out is allowed (the
