@@ -386,7 +386,7 @@ def suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs):
     for B in (2, 16, 256, 4096):
         for dist, dname in [(0, "uniform"), (1, "zipf")]:
             ctx.gen_split_keys(src, b, B, dist, 0x5917 + B)
-            t = T.time(lambda: ctx.split(src, ko, B, b, vo, start, n, edpp.SPLIT_WG), reps=5)
+            t = T.time(lambda: ctx.split(src, ko, B, b, vo, start, n, edpp.SPLIT_AUTO), reps=5)
             g = gbs(16.0 * n, t["hampel_ms"])
             sp[f"B{B}_{dname}"] = {"gbs": round(g, 2), "ms": round(t["hampel_ms"], 3),
                                    "frac_copy": round(g / copy_gbs, 4)}

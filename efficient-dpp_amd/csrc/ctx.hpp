@@ -40,7 +40,7 @@ struct edpp_ctx {
         t.scan_vpt = 16;
         t.scan_block = 512;
         t.split_ipt = 16;
-        t.split_tile_mult = 64;
+        t.split_tile_mult = 16;
         return t;
     }
 };
