@@ -22,26 +22,39 @@
 
 namespace edpp {
 
-template <class K, int IPT, bool BALLOT>
+// Per-tile bucket counts his[b*tiles + t].  A workgroup counts TPW
+// consecutive tiles (TPW > 1 only when a tile is one sub-tile), so that every
+// lane has TPW*IPT loads in flight and the workgroup's fixed costs (LDS
+// clearing, barriers, the column writes) are shared by TPW tiles; that pays
+// where the per-key LDS atomics dominate (many buckets).
+template <class K, int IPT, int TPW, int REGW>
 __global__ __launch_bounds__(kSplitBlock) void k_split_hist(const K* __restrict__ kin, SplitGeom g,
                                                             int32_t* __restrict__ his) {
     extern __shared__ uint32_t lds[];
     const int B = 1 << g.lg;
-    uint32_t* whist = lds;                     // [waves][B]
     constexpr int S = kSplitBlock * IPT;
-    const int64_t tile = xcd_tile(blockIdx.x, g.tiles, g.xcds);
-    const int64_t tbase = tile * g.tile;
-    const int64_t tend = std::min<int64_t>(tbase + g.tile, g.n);
-    const int w = threadIdx.x / kWave;
-    for (int b = threadIdx.x; b < kSplitWaves * B; b += kSplitBlock) whist[b] = 0;
+    uint32_t* whist = lds;  // [TPW][waves][B]
+    const int64_t groups = (g.tiles + TPW - 1) / TPW;
+    const int64_t t0 = xcd_tile(blockIdx.x, groups, g.xcds) * TPW;
+    for (int b = threadIdx.x; b < TPW * kSplitWaves * B; b += kSplitBlock) whist[b] = 0;
     __syncthreads();
-    for (int64_t s = tbase; s < tend; s += S)
-        count_subtile<K, IPT, BALLOT>(kin, s, tend, g.lg, g.shift, whist + w * B);
+    if (TPW > 1) {
+        count_tiles<K, IPT, TPW, REGW>(kin, t0 * S, g.n, g.lg, g.shift, whist);
+    } else {
+        const int64_t tbase = t0 * g.tile;
+        const int64_t tend = std::min<int64_t>(tbase + g.tile, g.n);
+        for (int64_t s = tbase; s < tend; s += S) count_tiles<K, IPT, 1, REGW>(kin, s, tend, g.lg, g.shift, whist);
+    }
     __syncthreads();
-    for (int b = threadIdx.x; b < B; b += kSplitBlock) {
-        uint32_t a = 0;
-        for (int q = 0; q < kSplitWaves; ++q) a += whist[q * B + b];
-        his[(int64_t)b * g.tiles + tile] = (int32_t)a;
+    // bucket-major: consecutive threads write consecutive tiles of one bucket
+    for (int i = threadIdx.x; i < TPW * B; i += kSplitBlock) {
+        const int b = i / TPW, t = i % TPW;
+        if (t0 + t < g.tiles) {
+            uint32_t a = 0;
+            #pragma unroll
+            for (int q = 0; q < kSplitWaves; ++q) a += whist[(t * kSplitWaves + q) * B + b];
+            his[(int64_t)b * g.tiles + t0 + t] = (int32_t)a;
+        }
     }
 }
 
@@ -205,6 +218,8 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
 // ------------------------------------------------------------- host side
 namespace {
 
+constexpr int kHistTilesPerWG = 4;
+
 template <int IPT, class K>
 size_t wg_lds_bytes(int B) {
     return (size_t)kbuf_word_offset(B) * 4 + (size_t)kSplitBlock * IPT * (sizeof(K) + 4);
@@ -261,16 +276,29 @@ int run_wg(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
     g.tiles = ceil_div(a.n, g.tile);
     g.xcds = c->xcds;
     const int64_t hlen = (int64_t)B * g.tiles;
-    size_t lh = (size_t)(kSplitWaves + 1) * B * 4;
+    // histogram: register counters up to 16 buckets; above, LDS atomics with
+    // four one-sub-tile tiles per workgroup where tiles are one sub-tile
+    const int regw = a.lg <= kRegCountBits ? reg_count_words(a.lg) : 0;
+    const int tpw = (regw == 0 && g.tile == S) ? kHistTilesPerWG : 1;
+    const size_t lh = (size_t)tpw * kSplitWaves * B * 4;
     size_t ls = wg_lds_bytes<IPT, K>(B);
-    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_hist<K, IPT, BALLOT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lh));
     EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_scatter<K, IPT, true, BALLOT>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls));
     EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_scatter<K, IPT, false, BALLOT>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls));
-    hipLaunchKernelGGL((k_split_hist<K, IPT, BALLOT>), dim3((unsigned)g.tiles), dim3(kSplitBlock), lh, c->stream, a.kin, g,
-                       his);
+    const unsigned hgrid = (unsigned)ceil_div(g.tiles, (int64_t)tpw);
+    switch (regw) {
+        case 1: hipLaunchKernelGGL((k_split_hist<K, IPT, 1, 1>), dim3(hgrid), dim3(kSplitBlock), lh, c->stream, a.kin, g, his); break;
+        case 2: hipLaunchKernelGGL((k_split_hist<K, IPT, 1, 2>), dim3(hgrid), dim3(kSplitBlock), lh, c->stream, a.kin, g, his); break;
+        case 4: hipLaunchKernelGGL((k_split_hist<K, IPT, 1, 4>), dim3(hgrid), dim3(kSplitBlock), lh, c->stream, a.kin, g, his); break;
+        default:
+            if (tpw > 1)
+                hipLaunchKernelGGL((k_split_hist<K, IPT, kHistTilesPerWG, 0>), dim3(hgrid), dim3(kSplitBlock), lh,
+                                   c->stream, a.kin, g, his);
+            else
+                hipLaunchKernelGGL((k_split_hist<K, IPT, 1, 0>), dim3(hgrid), dim3(kSplitBlock), lh, c->stream, a.kin,
+                                   g, his);
+    }
     EDPP_HIP(c, hipGetLastError());
     int rc = launch_scan<uint32_t>(c, (const uint32_t*)his, (uint32_t*)his, hlen, EDPP_SCAN_LOOKBACK, nullptr);
     if (rc) return rc;

@@ -25,6 +25,21 @@ __device__ __forceinline__ uint64_t match_bucket(uint32_t b, int lg, uint64_t va
 // whose same-address atomics serialize when many lanes share few buckets.
 constexpr int kBallotMatchBits = 4;
 
+// The lanes whose bucket equals that of the lowest valid lane, from one
+// ballot, and that bucket in *bf (0xFFFFFFFF when no lane is valid).  Under
+// skewed keys most lanes of a wave share one bucket; taking that group out of
+// the per-lane LDS atomics keeps them from serializing on one address.
+__device__ __forceinline__ uint64_t first_bucket_group(uint32_t b, bool ok, uint32_t* bf) {
+    const uint64_t okm = ballot(ok);
+    if (okm == 0) {
+        *bf = 0xFFFFFFFFu;
+        return 0;
+    }
+    const uint32_t f = read_lane(b, (int)__builtin_ctzll(okm));
+    *bf = f;
+    return ballot(ok && b == f);
+}
+
 // XCD-aware tile order: workgroups are dispatched round-robin over the XCDs,
 // so workgroup g runs on XCD g % xcds.  Give every XCD a contiguous range of
 // tiles, so that the write frontier of adjacent tiles (adjacent runs of the
@@ -95,7 +110,8 @@ __device__ __forceinline__ void load_subtile(const K* __restrict__ kin, const in
 
 // Bucket counts of the wave's chunk added to whist_w[b] (order-free).
 // BALLOT: one lane per group adds the group's size (few buckets, where LDS
-// atomics would collide); otherwise plain LDS atomics.
+// atomics would collide); otherwise per-lane LDS atomics, except for the
+// lowest valid lane's bucket group, which adds its size at once.
 template <class K, int IPT, bool BALLOT>
 __device__ __forceinline__ void count_subtile(const K* __restrict__ kin, int64_t sbase, int64_t n, int lg, int shift,
                                               uint32_t* whist_w) {
@@ -111,8 +127,115 @@ __device__ __forceinline__ void count_subtile(const K* __restrict__ kin, int64_t
             if (ok && count_below(m) == 0)
                 __hip_atomic_fetch_add(whist_w + bk[j], (uint32_t)__popcll(m), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WAVEFRONT);
-        } else if (ok) {
-            __hip_atomic_fetch_add(whist_w + bk[j], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        } else {
+            uint32_t bf;
+            const uint64_t mf = first_bucket_group(bk[j], ok, &bf);
+            if (ok && bk[j] == bf) {
+                if (count_below(mf) == 0)
+                    __hip_atomic_fetch_add(whist_w + bf, (uint32_t)__popcll(mf), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WAVEFRONT);
+            } else if (ok) {
+                __hip_atomic_fetch_add(whist_w + bk[j], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            }
+        }
+    }
+}
+
+// Histogram counting of the WG strategy's histogram kernel.  Up to this many
+// bucket bits every lane keeps its counts as 8-bit fields packed four to a
+// register (REGW = B/4 registers, at least 1), and the wave adds the fields up
+// once per sub-tile: no LDS traffic per key and no dependence on key skew
+// (LDS atomics serialize when many lanes hit few addresses).  Above it,
+// per-lane LDS atomics.
+constexpr int kRegCountBits = 4;
+__host__ __device__ constexpr int reg_count_words(int lg) { return lg <= 2 ? 1 : (1 << (lg - 2)); }
+
+// Counts TPW consecutive sub-tiles of S = 256*IPT elements, sub-tile t
+// starting at sbase + t*S (elements at or past n ignored), into
+// whist + (t*waves + w)*B.  Element layout as in load_subtile.  Every key load
+// of all TPW sub-tiles is issued before any is used.  REGW > 0 selects the
+// register counters (B <= 4*REGW), REGW == 0 LDS atomics.
+template <class K, int IPT, int TPW, int REGW>
+__device__ __forceinline__ void count_tiles(const K* __restrict__ kin, int64_t sbase, int64_t n, int lg, int shift,
+                                            uint32_t* whist) {
+    constexpr int S = kSplitBlock * IPT;
+    static_assert(IPT <= 255, "8-bit per-lane counters");
+    const int lane = lane_id();
+    const int w = threadIdx.x / kWave;
+    const int B = 1 << lg;
+    const uint32_t mask = (uint32_t)B - 1u;
+    const int64_t wbase = sbase + (int64_t)w * kWave * IPT + lane;
+    K key[TPW][IPT];
+    #pragma unroll
+    for (int t = 0; t < TPW; ++t)
+        #pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            const int64_t e = wbase + (int64_t)t * S + (int64_t)j * kWave;
+            key[t][j] = e < n ? kin[e] : K(0);
+        }
+    __builtin_amdgcn_sched_barrier(0);
+    #pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+        uint32_t* row = whist + (t * kSplitWaves + w) * B;
+        if constexpr (REGW > 0) {
+            uint32_t packed[REGW];  // bucket b: word b/4, bits [8(b%4), 8(b%4)+8)
+            #pragma unroll
+            for (int q = 0; q < REGW; ++q) packed[q] = 0;
+            #pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int64_t e = wbase + (int64_t)t * S + (int64_t)j * kWave;
+                const uint32_t b = (((uint32_t)key[t][j]) >> shift) & mask;
+                const uint32_t inc = e < n ? 1u << (8 * (b & 3u)) : 0u;
+                #pragma unroll
+                for (int q = 0; q < REGW; ++q) packed[q] += (b >> 2) == (uint32_t)q ? inc : 0u;
+            }
+            #pragma unroll
+            for (int q = 0; q < REGW; ++q) {
+                // two 16-bit fields per word: at most 64*IPT per bucket
+                const uint32_t even = wave_sum(packed[q] & 0x00FF00FFu);        // buckets 4q, 4q+2
+                const uint32_t odd = wave_sum((packed[q] >> 8) & 0x00FF00FFu);  // buckets 4q+1, 4q+3
+                if (lane == 0) {  // the row belongs to this wave alone
+                    const int b0 = 4 * q;
+                    if (b0 < B) row[b0] += even & 0xFFFFu;
+                    if (b0 + 1 < B) row[b0 + 1] += odd & 0xFFFFu;
+                    if (b0 + 2 < B) row[b0 + 2] += even >> 16;
+                    if (b0 + 3 < B) row[b0 + 3] += odd >> 16;
+                }
+            }
+        } else {
+            // skewed keys (an eighth of the wave or more in the first lane's
+            // bucket on the first or the middle item) take the grouped path
+            // for the whole sub-tile; the test is wave-uniform
+            uint32_t bf0, bf1;
+            const uint64_t mf0 = first_bucket_group((((uint32_t)key[t][0]) >> shift) & mask,
+                                                    wbase + (int64_t)t * S < n, &bf0);
+            const uint64_t mf1 = first_bucket_group((((uint32_t)key[t][IPT / 2]) >> shift) & mask,
+                                                    wbase + (int64_t)t * S + (int64_t)(IPT / 2) * kWave < n, &bf1);
+            if (__popcll(mf0) >= kWave / 8 || __popcll(mf1) >= kWave / 8) {
+                #pragma unroll
+                for (int j = 0; j < IPT; ++j) {
+                    const int64_t e = wbase + (int64_t)t * S + (int64_t)j * kWave;
+                    const bool ok = e < n;
+                    const uint32_t b = (((uint32_t)key[t][j]) >> shift) & mask;
+                    uint32_t bf;
+                    const uint64_t mf = first_bucket_group(b, ok, &bf);
+                    if (ok && b == bf) {
+                        if (count_below(mf) == 0)
+                            __hip_atomic_fetch_add(row + bf, (uint32_t)__popcll(mf), __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    } else if (ok) {
+                        __hip_atomic_fetch_add(row + b, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    }
+                }
+            } else {
+                #pragma unroll
+                for (int j = 0; j < IPT; ++j) {
+                    const int64_t e = wbase + (int64_t)t * S + (int64_t)j * kWave;
+                    if (e < n)
+                        __hip_atomic_fetch_add(row + ((((uint32_t)key[t][j]) >> shift) & mask), 1u, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+                }
+            }
         }
     }
 }
@@ -121,8 +244,9 @@ __device__ __forceinline__ void count_subtile(const K* __restrict__ kin, int64_t
 // of bucket b) + number of lanes below with the same bucket in item j.
 //
 // Phase 1 finds, for every item, the mask of lanes sharing the caller's
-// bucket: from bucket-bit ballots (BALLOT) or from wmask[b] (per wave, zero on
-// entry and on exit): every lane ORs in its own bit, reads the mask back and
+// bucket: from bucket-bit ballots (BALLOT), or from one ballot for the lowest
+// valid lane's bucket plus wmask[b] (per wave, zero on entry and on exit) for
+// the other lanes: each of them ORs in its own bit, reads the mask back and
 // clears it.  Phase 2 reads whist_w[b] and lets the lowest lane of each group
 // add the group size.  A wave's LDS instructions execute in program order, so
 // neither phase waits for an earlier item's result before issuing the next
@@ -142,10 +266,16 @@ __device__ __forceinline__ void rank_subtile(const K* __restrict__ kin, const in
         if (BALLOT) {
             m[j] = match_bucket(bk[j], lg, ballot(ok));
         } else {
+            // the lowest valid lane's bucket group comes from a ballot; the
+            // other lanes (none of them in that bucket) match through wmask
             const uint32_t b = ok ? bk[j] : 0u;
-            if (ok) __hip_atomic_fetch_or(wmask + b, 1ull << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-            m[j] = ok ? __hip_atomic_load(wmask + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) : 0ull;
-            if (ok) __hip_atomic_store(wmask + b, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            uint32_t bf;
+            const uint64_t mf = first_bucket_group(b, ok, &bf);
+            const bool lds = ok && b != bf;
+            if (lds) __hip_atomic_fetch_or(wmask + b, 1ull << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            const uint64_t ml = lds ? __hip_atomic_load(wmask + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) : 0ull;
+            if (lds) __hip_atomic_store(wmask + b, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            m[j] = lds ? ml : (ok ? mf : 0ull);
         }
     }
     #pragma unroll
