@@ -61,10 +61,12 @@ _SIGS = {
 class Tuning(ctypes.Structure):
     """edpp_tuning of include/edpp_tune.h (0 / -1 fields mean the default)."""
     _fields_ = [("gs_unroll", c_i32), ("gs_random_nt", c_i32), ("scan_vpt", c_i32), ("split_ipt", c_i32),
-                ("split_tile_mult", c_i32), ("scan_block", c_i32), ("reserved", c_i32 * 10)]
+                ("split_tile_mult", c_i32), ("scan_block", c_i32), ("split_block", c_i32),
+                ("reserved", c_i32 * 9)]
 
 
-TUNING_FIELDS = ("gs_unroll", "gs_random_nt", "scan_vpt", "split_ipt", "split_tile_mult", "scan_block")
+TUNING_FIELDS = ("gs_unroll", "gs_random_nt", "scan_vpt", "split_ipt", "split_tile_mult", "scan_block",
+                 "split_block")
 
 _SIGS["edpp_set_tuning"] = [c_vp, ctypes.POINTER(Tuning)]
 _SIGS["edpp_get_tuning"] = [c_vp, ctypes.POINTER(Tuning)]

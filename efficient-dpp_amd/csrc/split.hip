@@ -22,33 +22,35 @@
 
 namespace edpp {
 
-// Per-tile bucket counts his[b*tiles + t].  A workgroup counts TPW
-// consecutive tiles (TPW > 1 only when a tile is one sub-tile), so that every
-// lane has TPW*IPT loads in flight and the workgroup's fixed costs (LDS
-// clearing, barriers, the column writes) are shared by TPW tiles; that pays
-// where the per-key LDS atomics dominate (many buckets).
+// Per-tile bucket counts his[b*tiles + t], counted in sub-tiles of
+// S = 256*IPT elements (spt of them per tile).  With TPW > 1 a workgroup
+// counts TPW consecutive sub-tiles, that is TPW/spt whole tiles (spt divides
+// TPW), so that every lane has TPW*IPT loads in flight and the workgroup's
+// fixed costs (LDS clearing, barriers, the column writes) are shared; that
+// pays where the per-key LDS atomics dominate (many buckets).
 template <class K, int IPT, int TPW, int REGW>
-__global__ __launch_bounds__(kSplitBlock) void k_split_hist(const K* __restrict__ kin, SplitGeom g,
+__global__ __launch_bounds__(kSplitBlock) void k_split_hist(const K* __restrict__ kin, SplitGeom g, int spt,
                                                             int32_t* __restrict__ his) {
     extern __shared__ uint32_t lds[];
     const int B = 1 << g.lg;
     constexpr int S = kSplitBlock * IPT;
-    uint32_t* whist = lds;  // [TPW][waves][B]
-    const int64_t groups = (g.tiles + TPW - 1) / TPW;
-    const int64_t t0 = xcd_tile(blockIdx.x, groups, g.xcds) * TPW;
-    for (int b = threadIdx.x; b < TPW * kSplitWaves * B; b += kSplitBlock) whist[b] = 0;
+    const int tpw = TPW > 1 ? TPW / spt : 1;  // whole tiles per workgroup
+    uint32_t* whist = lds;                    // [tpw][waves][B]
+    const int64_t groups = (g.tiles + tpw - 1) / tpw;
+    const int64_t t0 = xcd_tile(blockIdx.x, groups, g.xcds) * tpw;
+    for (int b = threadIdx.x; b < tpw * kSplitWaves * B; b += kSplitBlock) whist[b] = 0;
     __syncthreads();
     if (TPW > 1) {
-        count_tiles<K, IPT, TPW, REGW>(kin, t0 * S, g.n, g.lg, g.shift, whist);
+        count_tiles<K, IPT, TPW, REGW>(kin, t0 * g.tile, g.n, g.lg, g.shift, spt, whist);
     } else {
         const int64_t tbase = t0 * g.tile;
         const int64_t tend = std::min<int64_t>(tbase + g.tile, g.n);
-        for (int64_t s = tbase; s < tend; s += S) count_tiles<K, IPT, 1, REGW>(kin, s, tend, g.lg, g.shift, whist);
+        for (int64_t s = tbase; s < tend; s += S) count_tiles<K, IPT, 1, REGW>(kin, s, tend, g.lg, g.shift, 1, whist);
     }
     __syncthreads();
     // bucket-major: consecutive threads write consecutive tiles of one bucket
-    for (int i = threadIdx.x; i < TPW * B; i += kSplitBlock) {
-        const int b = i / TPW, t = i % TPW;
+    for (int i = threadIdx.x; i < tpw * B; i += kSplitBlock) {
+        const int b = i / tpw, t = i % tpw;
         if (t0 + t < g.tiles) {
             uint32_t a = 0;
             #pragma unroll
@@ -58,30 +60,35 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_hist(const K* __restrict_
     }
 }
 
-template <class K, int IPT, bool KV, bool BALLOT>
-__global__ __launch_bounds__(kSplitBlock) void k_split_scatter(const K* __restrict__ kin, K* __restrict__ kout,
-                                                               const int32_t* __restrict__ vin,
-                                                               int32_t* __restrict__ vout, SplitGeom g,
-                                                               const int32_t* __restrict__ scanned) {
+// BLOCK threads; sub-tile S = BLOCK*IPT.  Above 256 threads the values are
+// staged after the keys through one buffer (place_and_store's SEP), and the
+// register budget is held to two workgroups per CU (4 waves per SIMD), which
+// their LDS allows.
+template <class K, int IPT, bool KV, bool BALLOT, int BLOCK>
+__global__ __launch_bounds__(BLOCK, (BLOCK > kSplitBlock ? 2 * BLOCK / (4 * kWave) : 1)) void k_split_scatter(const K* __restrict__ kin, K* __restrict__ kout,
+                                                         const int32_t* __restrict__ vin, int32_t* __restrict__ vout,
+                                                         SplitGeom g, const int32_t* __restrict__ scanned) {
     extern __shared__ uint32_t lds[];
+    constexpr int W = BLOCK / kWave;
+    constexpr bool SEP = KV && BLOCK > kSplitBlock;
     const int B = 1 << g.lg;
-    constexpr int S = kSplitBlock * IPT;
+    constexpr int S = BLOCK * IPT;
     uint64_t* wmask = reinterpret_cast<uint64_t*>(lds);  // [waves][B], zero between items
-    uint32_t* whist = lds + whist_word_offset(B);        // [waves][B]
-    uint32_t* run = whist + kSplitWaves * B;             // [B] next output position of each bucket
+    uint32_t* whist = lds + whist_word_offset(B, W);     // [waves][B]
+    uint32_t* run = whist + W * B;               // [B] next output position of each bucket
     uint32_t* sub = run + B;                     // [B] sub-tile bucket counts
     uint32_t* off = sub + B;                     // [B] sub-tile bucket offsets
     uint32_t* wtmp = off + B;                    // [waves]
-    K* kbuf = reinterpret_cast<K*>(lds + kbuf_word_offset(B));  // [S], 16-byte aligned
-    int32_t* vbuf = reinterpret_cast<int32_t*>(kbuf + S);
+    K* kbuf = reinterpret_cast<K*>(lds + kbuf_word_offset(B, W));  // [S], 16-byte aligned
+    int32_t* vbuf = SEP ? reinterpret_cast<int32_t*>(kbuf) : reinterpret_cast<int32_t*>(kbuf + S);
     const int64_t tile = xcd_tile(blockIdx.x, g.tiles, g.xcds);
     const int64_t tbase = tile * g.tile;
     const int64_t tend = std::min<int64_t>(tbase + g.tile, g.n);
     const int w = threadIdx.x / kWave;
-    for (int b = threadIdx.x; b < B; b += kSplitBlock) run[b] = (uint32_t)scanned[(int64_t)b * g.tiles + tile];
-    for (int b = threadIdx.x; b < kSplitWaves * B; b += kSplitBlock) wmask[b] = 0ull;
+    for (int b = threadIdx.x; b < B; b += BLOCK) run[b] = (uint32_t)scanned[(int64_t)b * g.tiles + tile];
+    for (int b = threadIdx.x; b < W * B; b += BLOCK) wmask[b] = 0ull;
     for (int64_t s = tbase; s < tend; s += S) {
-        for (int b = threadIdx.x; b < kSplitWaves * B; b += kSplitBlock) whist[b] = 0;
+        for (int b = threadIdx.x; b < W * B; b += BLOCK) whist[b] = 0;
         __syncthreads();
         uint32_t bk[IPT], rk[IPT];
         K key[IPT];
@@ -89,8 +96,8 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_scatter(const K* __restri
         rank_subtile<K, IPT, BALLOT>(kin, KV ? vin : nullptr, s, tend, g.lg, g.shift, whist + w * B, wmask + w * B,
                                      bk, rk, key, val);
         __syncthreads();
-        place_and_store<K, IPT, KV>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk, key, val, B, g.shift,
-                                    (int)std::min<int64_t>((int64_t)S, tend - s), kout, vout, g.n);
+        place_and_store<K, IPT, KV, BLOCK, SEP>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk, key, val, B, g.shift,
+                                                (int)std::min<int64_t>((int64_t)S, tend - s), kout, vout, g.n);
         // the next iteration's first barrier ends every thread's stores
         // before kbuf, sub and run are used again
     }
@@ -220,9 +227,11 @@ namespace {
 
 constexpr int kHistTilesPerWG = 4;
 
-template <int IPT, class K>
+template <int IPT, class K, int BLOCK = kSplitBlock>
 size_t wg_lds_bytes(int B) {
-    return (size_t)kbuf_word_offset(B) * 4 + (size_t)kSplitBlock * IPT * (sizeof(K) + 4);
+    constexpr bool SEP = BLOCK > kSplitBlock;  // the key-value kernel stages values after keys
+    const size_t per = SEP ? std::max<size_t>(sizeof(K), 4) : sizeof(K) + 4;
+    return (size_t)kbuf_word_offset(B, BLOCK / kWave) * 4 + (size_t)BLOCK * IPT * per;
 }
 
 template <class K>
@@ -255,19 +264,30 @@ int wg_ipt(edpp_ctx* c, int lg) {
     return (c->tune.split_ipt >= 16 && wg_lds_bytes<16, K>(1 << lg) <= 160 * 1024) ? 16 : 8;
 }
 
+// Lanes per scatter workgroup.  512 (sub-tiles of 8192 keys, values staged
+// after keys) from 128 buckets on: a bucket's run per sub-tile doubles, so
+// fewer stores are partial cache lines; only for 4-byte keys at 16 per lane.
+template <class K>
+int wg_block(edpp_ctx* c, int lg) {
+    if (sizeof(K) != 4 || wg_ipt<K>(c, lg) != 16 || wg_lds_bytes<16, K, 512>(1 << lg) > 160 * 1024) return kSplitBlock;
+    if (c->tune.split_block != 0) return c->tune.split_block;
+    return lg >= 7 ? 512 : kSplitBlock;
+}
+
 // histogram entries one pass needs
 template <class K>
 int64_t his_len(edpp_ctx* c, int strategy, int64_t n, int lg) {
     const int64_t B = (int64_t)1 << lg;
-    if (strategy == EDPP_SPLIT_WG) return B * ceil_div(n, wg_tile(c, lg, kSplitBlock * wg_ipt<K>(c, lg)));
+    if (strategy == EDPP_SPLIT_WG) return B * ceil_div(n, wg_tile(c, lg, wg_block<K>(c, lg) * wg_ipt<K>(c, lg)));
     if (strategy == EDPP_SPLIT_WI) return B * ceil_div(n, wi_colen(c, lg, 16));
     return B * single_lanes(c, n, lg);
 }
 
-template <class K, int IPT, bool BALLOT>
+template <class K, int IPT, bool BALLOT, int BLOCK>
 int run_wg(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
     const int B = 1 << a.lg;
-    constexpr int S = kSplitBlock * IPT;
+    constexpr int S = BLOCK * IPT;         // scatter sub-tile
+    constexpr int SH = kSplitBlock * IPT;  // histogram sub-tile
     SplitGeom g;
     g.n = a.n;
     g.lg = a.lg;
@@ -277,27 +297,30 @@ int run_wg(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
     g.xcds = c->xcds;
     const int64_t hlen = (int64_t)B * g.tiles;
     // histogram: register counters up to 16 buckets; above, LDS atomics with
-    // four one-sub-tile tiles per workgroup where tiles are one sub-tile
+    // four histogram sub-tiles per workgroup where they make whole tiles
     const int regw = a.lg <= kRegCountBits ? reg_count_words(a.lg) : 0;
-    const int tpw = (regw == 0 && g.tile == S) ? kHistTilesPerWG : 1;
+    const int64_t spt = g.tile / SH;
+    const bool multi = regw == 0 && spt <= kHistTilesPerWG && (spt & (spt - 1)) == 0;
+    const int tpw = multi ? kHistTilesPerWG / (int)spt : 1;
     const size_t lh = (size_t)tpw * kSplitWaves * B * 4;
-    size_t ls = wg_lds_bytes<IPT, K>(B);
-    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_scatter<K, IPT, true, BALLOT>,
+    const size_t ls = wg_lds_bytes<IPT, K, BLOCK>(B);
+    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_scatter<K, IPT, true, BALLOT, BLOCK>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls));
-    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_scatter<K, IPT, false, BALLOT>,
+    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_scatter<K, IPT, false, BALLOT, BLOCK>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls));
     const unsigned hgrid = (unsigned)ceil_div(g.tiles, (int64_t)tpw);
+    const int ispt = (int)spt;
     switch (regw) {
-        case 1: hipLaunchKernelGGL((k_split_hist<K, IPT, 1, 1>), dim3(hgrid), dim3(kSplitBlock), lh, c->stream, a.kin, g, his); break;
-        case 2: hipLaunchKernelGGL((k_split_hist<K, IPT, 1, 2>), dim3(hgrid), dim3(kSplitBlock), lh, c->stream, a.kin, g, his); break;
-        case 4: hipLaunchKernelGGL((k_split_hist<K, IPT, 1, 4>), dim3(hgrid), dim3(kSplitBlock), lh, c->stream, a.kin, g, his); break;
+        case 1: hipLaunchKernelGGL((k_split_hist<K, IPT, 1, 1>), dim3(hgrid), dim3(kSplitBlock), lh, c->stream, a.kin, g, 1, his); break;
+        case 2: hipLaunchKernelGGL((k_split_hist<K, IPT, 1, 2>), dim3(hgrid), dim3(kSplitBlock), lh, c->stream, a.kin, g, 1, his); break;
+        case 4: hipLaunchKernelGGL((k_split_hist<K, IPT, 1, 4>), dim3(hgrid), dim3(kSplitBlock), lh, c->stream, a.kin, g, 1, his); break;
         default:
-            if (tpw > 1)
+            if (multi)
                 hipLaunchKernelGGL((k_split_hist<K, IPT, kHistTilesPerWG, 0>), dim3(hgrid), dim3(kSplitBlock), lh,
-                                   c->stream, a.kin, g, his);
+                                   c->stream, a.kin, g, ispt, his);
             else
                 hipLaunchKernelGGL((k_split_hist<K, IPT, 1, 0>), dim3(hgrid), dim3(kSplitBlock), lh, c->stream, a.kin,
-                                   g, his);
+                                   g, 1, his);
     }
     EDPP_HIP(c, hipGetLastError());
     int rc = launch_scan<uint32_t>(c, (const uint32_t*)his, (uint32_t*)his, hlen, EDPP_SCAN_LOOKBACK, nullptr);
@@ -306,10 +329,10 @@ int run_wg(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
         hipLaunchKernelGGL(k_split_start, dim3((unsigned)ceil_div(B, 256)), dim3(256), 0, c->stream,
                            (const int32_t*)his, g.tiles, B, a.d_start);
     if (a.vin)
-        hipLaunchKernelGGL((k_split_scatter<K, IPT, true, BALLOT>), dim3((unsigned)g.tiles), dim3(kSplitBlock), ls, c->stream,
-                           a.kin, a.kout, a.vin, a.vout, g, (const int32_t*)his);
+        hipLaunchKernelGGL((k_split_scatter<K, IPT, true, BALLOT, BLOCK>), dim3((unsigned)g.tiles), dim3(BLOCK), ls,
+                           c->stream, a.kin, a.kout, a.vin, a.vout, g, (const int32_t*)his);
     else
-        hipLaunchKernelGGL((k_split_scatter<K, IPT, false, BALLOT>), dim3((unsigned)g.tiles), dim3(kSplitBlock), ls,
+        hipLaunchKernelGGL((k_split_scatter<K, IPT, false, BALLOT, BLOCK>), dim3((unsigned)g.tiles), dim3(BLOCK), ls,
                            c->stream, a.kin, a.kout, a.vin, a.vout, g, (const int32_t*)his);
     EDPP_HIP(c, hipGetLastError());
     return EDPP_OK;
@@ -378,9 +401,14 @@ int run_pass(edpp_ctx* c, int strategy, const SplitCall<K>& a, int32_t* his) {
     switch (strategy) {
         case EDPP_SPLIT_WG: {
             const bool bal = a.lg <= kBallotMatchBits;
-            if (wg_ipt<K>(c, a.lg) == 16)
-                return bal ? run_wg<K, 16, true>(c, a, his) : run_wg<K, 16, false>(c, a, his);
-            return bal ? run_wg<K, 8, true>(c, a, his) : run_wg<K, 8, false>(c, a, his);
+            if (wg_ipt<K>(c, a.lg) == 16) {
+                if constexpr (sizeof(K) == 4) {
+                    if (wg_block<K>(c, a.lg) == 512)
+                        return bal ? run_wg<K, 16, true, 512>(c, a, his) : run_wg<K, 16, false, 512>(c, a, his);
+                }
+                return bal ? run_wg<K, 16, true, kSplitBlock>(c, a, his) : run_wg<K, 16, false, kSplitBlock>(c, a, his);
+            }
+            return bal ? run_wg<K, 8, true, kSplitBlock>(c, a, his) : run_wg<K, 8, false, kSplitBlock>(c, a, his);
         }
         case EDPP_SPLIT_WI:
             return a.lg <= kBallotMatchBits ? run_wi<K, 16, true>(c, a, his) : run_wi<K, 16, false>(c, a, his);

@@ -55,10 +55,11 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t g, int64_t tiles, int xcds) 
 }
 // LDS layout of the WG scatter (32-bit words): wmask[waves][B] (64-bit),
 // whist[waves][B], run[B], sub[B], off[B], wtmp[16], then the key and value
-// reorder buffers from a 16-byte boundary
-__host__ __device__ __forceinline__ int whist_word_offset(int B) { return 2 * kSplitWaves * B; }
-__host__ __device__ __forceinline__ int kbuf_word_offset(int B) {
-    return ((whist_word_offset(B) + (kSplitWaves + 3) * B + 16) + 3) & ~3;
+// reorder buffers from a 16-byte boundary (the value buffer aliases the key
+// buffer when values are staged after keys)
+__host__ __device__ __forceinline__ int whist_word_offset(int B, int waves = kSplitWaves) { return 2 * waves * B; }
+__host__ __device__ __forceinline__ int kbuf_word_offset(int B, int waves = kSplitWaves) {
+    return ((whist_word_offset(B, waves) + (waves + 3) * B + 16) + 3) & ~3;
 }
 
 // ------------------------------------------------------------- WG strategy
@@ -151,13 +152,14 @@ constexpr int kRegCountBits = 4;
 __host__ __device__ constexpr int reg_count_words(int lg) { return lg <= 2 ? 1 : (1 << (lg - 2)); }
 
 // Counts TPW consecutive sub-tiles of S = 256*IPT elements, sub-tile t
-// starting at sbase + t*S (elements at or past n ignored), into
-// whist + (t*waves + w)*B.  Element layout as in load_subtile.  Every key load
-// of all TPW sub-tiles is issued before any is used.  REGW > 0 selects the
-// register counters (B <= 4*REGW), REGW == 0 LDS atomics.
+// starting at sbase + t*S (elements at or past n ignored), into the rows of
+// tile t/spt: whist + ((t/spt)*waves + w)*B.  Element layout as in
+// load_subtile.  Every key load of all TPW sub-tiles is issued before any is
+// used.  REGW > 0 selects the register counters (B <= 4*REGW), REGW == 0 LDS
+// atomics.
 template <class K, int IPT, int TPW, int REGW>
 __device__ __forceinline__ void count_tiles(const K* __restrict__ kin, int64_t sbase, int64_t n, int lg, int shift,
-                                            uint32_t* whist) {
+                                            int spt, uint32_t* whist) {
     constexpr int S = kSplitBlock * IPT;
     static_assert(IPT <= 255, "8-bit per-lane counters");
     const int lane = lane_id();
@@ -176,7 +178,7 @@ __device__ __forceinline__ void count_tiles(const K* __restrict__ kin, int64_t s
     __builtin_amdgcn_sched_barrier(0);
     #pragma unroll
     for (int t = 0; t < TPW; ++t) {
-        uint32_t* row = whist + (t * kSplitWaves + w) * B;
+        uint32_t* row = whist + ((t / spt) * kSplitWaves + w) * B;
         if constexpr (REGW > 0) {
             uint32_t packed[REGW];  // bucket b: word b/4, bits [8(b%4), 8(b%4)+8)
             #pragma unroll
@@ -292,10 +294,13 @@ __device__ __forceinline__ void rank_subtile(const K* __restrict__ kin, const in
     }
 }
 
-// exclusive scan over the B bucket totals held in cnt[] into off[] (block-wide)
+// exclusive scan over the B bucket totals held in cnt[] into off[] (block-wide,
+// BLOCK threads)
+template <int BLOCK = kSplitBlock>
 __device__ __forceinline__ void scan_buckets(const uint32_t* cnt, uint32_t* off, int B, uint32_t* wtmp) {
-    // each lane takes a run of per = ceil(B/256) consecutive buckets
-    const int per = (B + kSplitBlock - 1) / kSplitBlock;
+    constexpr int W = BLOCK / kWave;
+    // each lane takes a run of per = ceil(B/BLOCK) consecutive buckets
+    const int per = (B + BLOCK - 1) / BLOCK;
     const int b0 = threadIdx.x * per;
     uint32_t s = 0;
     for (int q = 0; q < per; ++q)
@@ -305,7 +310,7 @@ __device__ __forceinline__ void scan_buckets(const uint32_t* cnt, uint32_t* off,
     if (lane_id() == kWave - 1) wtmp[w] = inc;
     __syncthreads();
     uint32_t before = 0;
-    for (int q = 0; q < kSplitWaves; ++q) before += q < w ? wtmp[q] : 0u;
+    for (int q = 0; q < W; ++q) before += q < w ? wtmp[q] : 0u;
     uint32_t x = before + inc - s;
     for (int q = 0; q < per; ++q)
         if (b0 + q < B) {
@@ -315,40 +320,45 @@ __device__ __forceinline__ void scan_buckets(const uint32_t* cnt, uint32_t* off,
         }
 }
 
-// Second half of a WG-strategy sub-tile, after rank_subtile: per-wave counts
-// become sub-tile offsets, the sub-tile is reordered by bucket in LDS (kbuf,
-// vbuf), and slot i of bucket b is written to run[b] + (i - off[b]), so the
-// consecutive slots of a bucket leave as consecutive stores.  run[b] (each
-// bucket's next output position) advances by the sub-tile's count of b; sub[]
-// is reused for run[b] - off[b].  cnt is the number of valid elements (S for
-// every sub-tile but the last).  The stream-out is a plain loop: each store
-// leaves as soon as its slot is read, which measured faster than reading
-// every slot first and storing in one burst.  Block of kSplitBlock threads;
-// ends without a barrier.
-template <class K, int IPT, bool KV>
+// Second half of a WG-strategy sub-tile (S = BLOCK*IPT elements), after
+// rank_subtile: per-wave counts become sub-tile offsets, the sub-tile is
+// reordered by bucket in LDS, and slot i of bucket b is written to
+// run[b] + (i - off[b]), so the consecutive slots of a bucket leave as
+// consecutive stores.  run[b] (each bucket's next output position) advances by
+// the sub-tile's count of b; sub[] is reused for run[b] - off[b].  cnt is the
+// number of valid elements (S for every sub-tile but the last).
+//
+// SEP: the values follow the keys through the same buffer (vbuf == kbuf), so
+// a sub-tile twice as long fits the LDS of one staged with its keys; longer
+// bucket runs leave as fewer partial-line stores.  Otherwise keys and values
+// are staged together and streamed out by a plain loop (each store leaves as
+// soon as its slot is read, which measured faster than one burst).  Ends
+// without a barrier.
+template <class K, int IPT, bool KV, int BLOCK = kSplitBlock, bool SEP = false>
 __device__ __forceinline__ void place_and_store(uint32_t* whist, uint32_t* run, uint32_t* sub, uint32_t* off,
                                                 uint32_t* wtmp, K* kbuf, int32_t* vbuf, const uint32_t (&bk)[IPT],
                                                 const uint32_t (&rk)[IPT], const K (&key)[IPT],
                                                 const int32_t (&val)[IPT], int B, int shift, int cnt,
                                                 K* __restrict__ kout, int32_t* __restrict__ vout, int64_t n) {
-    constexpr int S = kSplitBlock * IPT;
+    constexpr int W = BLOCK / kWave;
+    constexpr int S = BLOCK * IPT;
     const int w = threadIdx.x / kWave;
     const uint32_t mask = (uint32_t)B - 1u;
     // wave prefixes per bucket (in place) and sub-tile totals
-    for (int b = threadIdx.x; b < B; b += kSplitBlock) {
-        uint32_t x[kSplitWaves];
+    for (int b = threadIdx.x; b < B; b += BLOCK) {
+        uint32_t x[W];
         #pragma unroll
-        for (int q = 0; q < kSplitWaves; ++q) x[q] = whist[q * B + b];
+        for (int q = 0; q < W; ++q) x[q] = whist[q * B + b];
         uint32_t a = 0;
         #pragma unroll
-        for (int q = 0; q < kSplitWaves; ++q) {
+        for (int q = 0; q < W; ++q) {
             whist[q * B + b] = a;
             a += x[q];
         }
         sub[b] = a;
     }
     __syncthreads();
-    scan_buckets(sub, off, B, wtmp);
+    scan_buckets<BLOCK>(sub, off, B, wtmp);
     __syncthreads();
     uint32_t pos[IPT];
     #pragma unroll
@@ -358,24 +368,46 @@ __device__ __forceinline__ void place_and_store(uint32_t* whist, uint32_t* run, 
     for (int j = 0; j < IPT; ++j) {
         if (pos[j] < (uint32_t)S) {  // false only past n, for consistent ranks
             kbuf[pos[j]] = key[j];
-            if (KV) vbuf[pos[j]] = val[j];
+            if (KV && !SEP) vbuf[pos[j]] = val[j];
         }
     }
     // each thread owns its buckets here: nobody reads sub[] or run[] again
     // before the barrier below
-    for (int b = threadIdx.x; b < B; b += kSplitBlock) {
+    for (int b = threadIdx.x; b < B; b += BLOCK) {
         const uint32_t c = sub[b];
         sub[b] = run[b] - off[b];
         run[b] += c;
     }
     __syncthreads();
     const uint32_t* dlt = sub;
-    for (int i = threadIdx.x; i < cnt; i += kSplitBlock) {
-        const K k = kbuf[i];
-        const uint32_t p = dlt[((uint32_t)k >> shift) & mask] + (uint32_t)i;
-        if ((int64_t)p < n) {  // always true for a consistent histogram scan
-            kout[p] = k;
-            if (KV) vout[p] = vbuf[i];
+    if constexpr (SEP && KV) {
+        // slot tid + j*BLOCK; its output position is kept for the value round
+        uint32_t d[IPT];
+        #pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            const int i = threadIdx.x + j * BLOCK;
+            const K k = kbuf[i < cnt ? i : 0];
+            d[j] = i < cnt ? dlt[((uint32_t)k >> shift) & mask] + (uint32_t)i : 0xFFFFFFFFu;
+            if ((int64_t)d[j] < n) kout[d[j]] = k;  // false only for empty slots
+        }
+        __syncthreads();  // every key read before the values overwrite the buffer
+        #pragma unroll
+        for (int j = 0; j < IPT; ++j)
+            if (pos[j] < (uint32_t)S) vbuf[pos[j]] = val[j];
+        __syncthreads();
+        #pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            const int i = threadIdx.x + j * BLOCK;
+            if ((int64_t)d[j] < n) vout[d[j]] = vbuf[i];
+        }
+    } else {
+        for (int i = threadIdx.x; i < cnt; i += BLOCK) {
+            const K k = kbuf[i];
+            const uint32_t p = dlt[((uint32_t)k >> shift) & mask] + (uint32_t)i;
+            if ((int64_t)p < n) {  // always true for a consistent histogram scan
+                kout[p] = k;
+                if (KV) vout[p] = vbuf[i];
+            }
         }
     }
 }

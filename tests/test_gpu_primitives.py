@@ -261,13 +261,15 @@ def test_split_i32(torch_gpu, ctx, orc, strategy, B, n):
         assert np.array_equal(host(kout2), ko)
 
 
-@pytest.mark.parametrize("ipt,mult", [(8, 16), (8, 64), (16, 16), (16, 64), (16, 256)])
+@pytest.mark.parametrize("ipt,mult,block", [(8, 16, 0), (8, 64, 0), (16, 16, 0), (16, 64, 0), (16, 256, 0),
+                                            (16, 16, 256), (16, 16, 512), (16, 64, 512), (16, 96, 512)])
 @pytest.mark.parametrize("B", [2, 16, 128, 256, 4096])
-def test_split_wg_shapes(torch_gpu, ctx, orc, ipt, mult, B):
-    # sub-tile shapes of the WG strategy, sizes ragged against both the
-    # sub-tile and the tile
+def test_split_wg_shapes(torch_gpu, ctx, orc, ipt, mult, block, B):
+    # sub-tile shapes of the WG strategy (256- and 512-lane scatter workgroups,
+    # tiles of 1..several sub-tiles), sizes ragged against both the sub-tile
+    # and the tile
     torch = torch_gpu
-    ctx.set_tuning(split_ipt=ipt, split_tile_mult=mult)
+    ctx.set_tuning(split_ipt=ipt, split_tile_mult=mult, split_block=block)
     try:
         for n in (8192 * 3 + 4096 + 77, 200_003):
             keys, vals = orc.gen_split_keys(n, B, n % 2, 77 + B + ipt)
@@ -277,9 +279,9 @@ def test_split_wg_shapes(torch_gpu, ctx, orc, ipt, mult, B):
             vout = empty(torch, n, torch.int32)
             st = empty(torch, B, torch.int32)
             ctx.split(dk, kout, B, dvv, vout, st, n, 1)
-            assert np.array_equal(host(kout), ko), (ipt, mult, B, n)
-            assert np.array_equal(host(vout), vo), (ipt, mult, B, n)
-            assert np.array_equal(host(st), start), (ipt, mult, B, n)
+            assert np.array_equal(host(kout), ko), (ipt, mult, block, B, n)
+            assert np.array_equal(host(vout), vo), (ipt, mult, block, B, n)
+            assert np.array_equal(host(st), start), (ipt, mult, block, B, n)
             kout2 = empty(torch, n, torch.int32)
             ctx.split(dk, kout2, B, n=n, strategy=1)
             assert np.array_equal(host(kout2), ko), (ipt, mult, B, n)
