@@ -392,6 +392,19 @@ def suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs):
                                    "frac_copy": round(g / copy_gbs, 4)}
             print("split", B, dname, sp[f"B{B}_{dname}"], file=sys.stderr, flush=True)
     res["split_kv"] = sp
+    # LSD radix sort on the split: 2^28 (int32 key, int32 value) pairs, all 32 key bits
+    ctx.gen_i32(src, 0x50F7)
+    b.copy_(torch.arange(n, dtype=i32, device="cuda"))
+    srt = {}
+    for name, fn in [("pairs_u32", lambda: ctx.sort(src, ko, b, vo, n, 0, 32)),
+                     ("pairs_i32", lambda: ctx.sort(src, ko, b, vo, n, 0, 32, True)),
+                     ("keys_u32", lambda: ctx.sort(src, ko, None, None, n, 0, 32))]:
+        t = T.time(fn, reps=5)
+        per = 16.0 if name.startswith("pairs") else 8.0  # one read and one write of each pair
+        srt[name] = {"ms": round(t["hampel_ms"], 3), "gkeys_s": round(n / (t["hampel_ms"] * 1e-3) / 1e9, 2),
+                     "gbs_one_pass_bytes": round(gbs(per * n, t["hampel_ms"]), 1)}
+    res["sort"] = srt
+    print("sort", srt, file=sys.stderr, flush=True)
     del ko, vo, b
     ctx.gen_i32(src, 0xDA7A)
     return res

@@ -45,6 +45,8 @@ _SIGS = {
     "edpp_gather_i64": [c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_dp],
     "edpp_scatter_i64": [c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_dp],
     "edpp_default_passes": [c_vp, c_i64, c_int, c_int, ctypes.POINTER(c_int)],
+    "edpp_sort_u32": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_dp],
+    "edpp_sort_i32": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_dp],
     "edpp_scan_exclusive_i32": [c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_dp],
     "edpp_scan_exclusive_i64": [c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_dp],
     "edpp_scan_local_i32": [c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_dp],
@@ -258,6 +260,14 @@ class Context:
         n = _numel(idx) if n is None else n
         name = "edpp_scatter_i64" if wide else "edpp_scatter_i32"
         return self._call(name, _ptr(src), _ptr(dst), _ptr(idx), n, int(passes), timed=timed)
+
+    def sort(self, keys_in, keys_out, vals_in=None, vals_out=None, n=None, begin_bit=0, end_bit=32, signed=False,
+             timed=False):
+        """Stable LSD radix sort of 32-bit keys by bits [begin_bit, end_bit); values follow."""
+        n = _numel(keys_in) if n is None else n
+        name = "edpp_sort_i32" if signed else "edpp_sort_u32"
+        return self._call(name, _ptr(keys_in), _ptr(keys_out), _ptr(vals_in), _ptr(vals_out), n, int(begin_bit),
+                          int(end_bit), timed=timed)
 
     def schedule_passes(self, n, scatter=False, wide=False):
         """Pass count a passes=0 gather/scatter of n elements uses without a cost model."""

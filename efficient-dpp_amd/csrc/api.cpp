@@ -455,6 +455,31 @@ int edpp_split_i64(edpp_ctx* c, const int64_t* kin, int64_t* kout, const int32_t
                                   ms);
 }
 
+// ---------------------------------------------------------------- radix sort
+static int sort_common(edpp_ctx* c, const uint32_t* kin, uint32_t* kout, const int32_t* vin, int32_t* vout,
+                       int64_t n, int begin_bit, int end_bit, bool is_signed, double* ms) {
+    EDPP_CHECK_CTX(c);
+    if (n < 0 || n >= (1ll << 31)) return fail(c, EDPP_EINVAL, "n=%lld outside [0, 2^31)", (long long)n);
+    if (n > 0 && (!kin || !kout)) return fail(c, EDPP_EINVAL, "null key buffer");
+    if ((vin == nullptr) != (vout == nullptr))
+        return fail(c, EDPP_EINVAL, "values: both or neither of vals_in/vals_out");
+    if (begin_bit < 0 || end_bit > 32 || begin_bit > end_bit)
+        return fail(c, EDPP_EINVAL, "key bits [%d, %d) outside [0, 32)", begin_bit, end_bit);
+    int rc = timing_begin(c, ms);
+    if (!rc) rc = launch_sort_u32(c, kin, kout, vin, vout, n, begin_bit, end_bit, is_signed);
+    if (!rc) rc = timing_end(c, ms);
+    return rc;
+}
+int edpp_sort_u32(edpp_ctx* c, const uint32_t* keys_in, uint32_t* keys_out, const int32_t* vals_in, int32_t* vals_out,
+                  int64_t n, int begin_bit, int end_bit, double* ms) {
+    return sort_common(c, keys_in, keys_out, vals_in, vals_out, n, begin_bit, end_bit, false, ms);
+}
+int edpp_sort_i32(edpp_ctx* c, const int32_t* keys_in, int32_t* keys_out, const int32_t* vals_in, int32_t* vals_out,
+                  int64_t n, int begin_bit, int end_bit, double* ms) {
+    return sort_common(c, (const uint32_t*)keys_in, (uint32_t*)keys_out, vals_in, vals_out, n, begin_bit, end_bit, true,
+                       ms);
+}
+
 // ---------------------------------------------------------------- generators
 int edpp_gen_i32(edpp_ctx* c, int32_t* out, int64_t n, uint64_t seed, int32_t mask) {
     EDPP_CHECK_CTX(c);

@@ -80,6 +80,8 @@ int launch_scan_local(edpp_ctx* c, const int32_t* in, int32_t* out, int tile, in
 template <class K>
 int launch_split(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t* vout,
                  int32_t* d_start, int64_t n, int buckets, int strategy);
+int launch_sort_u32(edpp_ctx* c, const uint32_t* kin, uint32_t* kout, const int32_t* vin, int32_t* vout, int64_t n,
+                    int begin_bit, int end_bit, bool is_signed);
 int launch_gen_u32(edpp_ctx* c, uint32_t* out, int64_t n, uint64_t seed, uint32_t mask);
 int launch_gen_u64(edpp_ctx* c, uint64_t* out, int64_t n, uint64_t seed, uint64_t mask);
 int launch_gen_perm(edpp_ctx* c, int32_t* idx, int64_t n, uint64_t seed, int kind);

@@ -117,4 +117,15 @@ double single_split(const int* d_in, int* d_out, int length, int buckets, bool r
     return split_common(d_in, d_out, nullptr, length, buckets, structure, nullptr, nullptr, EDPP_SPLIT_SINGLE);
 }
 
+double radixSort(int* d_source_keys, int* d_source_values, int len, bool isRecord) {
+    edpp_ctx* c = context();
+    if (!c) return -1.0;
+    double ms = 0.0;
+    if (isRecord)
+        return ms_or_fail(edpp_sort_i32(c, d_source_keys, d_source_keys, d_source_values, d_source_values, len, 0, 32,
+                                        &ms),
+                          ms);
+    return ms_or_fail(edpp_sort_i32(c, d_source_values, d_source_values, nullptr, nullptr, len, 0, 32, &ms), ms);
+}
+
 }  // namespace edpp

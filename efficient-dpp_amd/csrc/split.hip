@@ -41,11 +41,12 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_hist(const K* __restrict_
     for (int b = threadIdx.x; b < tpw * kSplitWaves * B; b += kSplitBlock) whist[b] = 0;
     __syncthreads();
     if (TPW > 1) {
-        count_tiles<K, IPT, TPW, REGW>(kin, t0 * g.tile, g.n, g.lg, g.shift, spt, whist);
+        count_tiles<K, IPT, TPW, REGW>(kin, t0 * g.tile, g.n, g.lg, g.shift, spt, whist, g.flip);
     } else {
         const int64_t tbase = t0 * g.tile;
         const int64_t tend = std::min<int64_t>(tbase + g.tile, g.n);
-        for (int64_t s = tbase; s < tend; s += S) count_tiles<K, IPT, 1, REGW>(kin, s, tend, g.lg, g.shift, 1, whist);
+        for (int64_t s = tbase; s < tend; s += S)
+            count_tiles<K, IPT, 1, REGW>(kin, s, tend, g.lg, g.shift, 1, whist, g.flip);
     }
     __syncthreads();
     // bucket-major: consecutive threads write consecutive tiles of one bucket
@@ -94,10 +95,10 @@ __global__ __launch_bounds__(BLOCK, (BLOCK > kSplitBlock ? 2 * BLOCK / (4 * kWav
         K key[IPT];
         int32_t val[IPT];
         rank_subtile<K, IPT, BALLOT>(kin, KV ? vin : nullptr, s, tend, g.lg, g.shift, whist + w * B, wmask + w * B,
-                                     bk, rk, key, val);
+                                     bk, rk, key, val, g.flip);
         __syncthreads();
         place_and_store<K, IPT, KV, BLOCK, SEP>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk, key, val, B, g.shift,
-                                                (int)std::min<int64_t>((int64_t)S, tend - s), kout, vout, g.n);
+                                                (int)std::min<int64_t>((int64_t)S, tend - s), kout, vout, g.n, g.flip);
         // the next iteration's first barrier ends every thread's stores
         // before kbuf, sub and run are used again
     }
@@ -243,6 +244,7 @@ struct SplitCall {
     int32_t* d_start;  // start offsets of the (lg, shift) buckets, may be null
     int64_t n;
     int lg, shift;
+    uint32_t flip;     // WG strategy only: bucket bits inverted (signed top digit of a radix sort)
 };
 
 int64_t wg_tile(edpp_ctx* c, int lg, int S) {
@@ -295,6 +297,7 @@ int run_wg(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
     g.tile = wg_tile(c, a.lg, S);
     g.tiles = ceil_div(a.n, g.tile);
     g.xcds = c->xcds;
+    g.flip = a.flip;
     const int64_t hlen = (int64_t)B * g.tiles;
     // histogram: register counters up to 16 buckets; above, LDS atomics with
     // four histogram sub-tiles per workgroup where they make whole tiles
@@ -429,7 +432,11 @@ int launch_split(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t
     int lg = 0;
     while ((1 << lg) < buckets) ++lg;
     if (n == 0) {
-        if (d_start) EDPP_HIP(c, hipMemsetAsync(d_start, 0, sizeof(int32_t) * buckets, c->stream));
+        // a fill kernel, not a memset node, so the call stays graph-capturable
+        if (d_start)
+            hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)ceil_div(buckets, 256)), dim3(256), 0, c->stream,
+                               (uint32_t*)d_start, (int64_t)buckets, 0u);
+        EDPP_HIP(c, hipGetLastError());
         return EDPP_OK;
     }
     if (strategy == EDPP_SPLIT_AUTO) {
@@ -456,15 +463,15 @@ int launch_split(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t
     char* base = static_cast<char*>(w);
     int32_t* his = reinterpret_cast<int32_t*>(base);
     if (!two) {
-        SplitCall<K> a = {kin, kout, vin, vout, d_start, n, lg, 0};
+        SplitCall<K> a = {kin, kout, vin, vout, d_start, n, lg, 0, 0u};
         return run_pass<K>(c, strategy, a, his);
     }
     K* tk = reinterpret_cast<K*>(base + hbytes);
     int32_t* tv = vin ? reinterpret_cast<int32_t*>(base + hbytes + kbytes) : nullptr;
-    SplitCall<K> a1 = {kin, tk, vin, tv, nullptr, n, lg1, 0};
+    SplitCall<K> a1 = {kin, tk, vin, tv, nullptr, n, lg1, 0, 0u};
     rc = run_pass<K>(c, strategy, a1, his);
     if (rc) return rc;
-    SplitCall<K> a2 = {tk, kout, tv, vout, nullptr, n, lg2, lg1};
+    SplitCall<K> a2 = {tk, kout, tv, vout, nullptr, n, lg2, lg1, 0u};
     rc = run_pass<K>(c, strategy, a2, his);
     if (rc) return rc;
     if (d_start) {
@@ -478,6 +485,88 @@ int launch_split(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t
         if (rc) return rc;
     }
     return EDPP_OK;
+}
+
+// Stable LSD radix sort of 32-bit keys (values follow) by key bits
+// [begin_bit, end_bit): one stable WG split per digit of at most 8 bits,
+// lowest digit first, as the reference's radix sort counts, scans and
+// scatters one digit per pass (obsolete/CUDA/kernels/radixSort.cu:29-200).
+// Signed keys order as int32: the digit holding bit 31 has its top bit
+// flipped.  Passes alternate between the output and workspace copies, so the
+// last one lands in the output; keys_in may equal keys_out.
+int launch_sort_u32(edpp_ctx* c, const uint32_t* kin, uint32_t* kout, const int32_t* vin, int32_t* vout, int64_t n,
+                    int begin_bit, int end_bit, bool is_signed) {
+    if (n == 0) return EDPP_OK;
+    if (begin_bit == end_bit) {  // no key bits: the input order is the sorted order
+        if (kout != kin) EDPP_HIP(c, hipMemcpyAsync(kout, kin, (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream));
+        if (vin && vout != vin)
+            EDPP_HIP(c, hipMemcpyAsync(vout, vin, (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream));
+        return EDPP_OK;
+    }
+    const int bits = end_bit - begin_bit;
+    const int passes = (bits + kMaxSplitBitsPerPass - 1) / kMaxSplitBitsPerPass;
+    const int per = (bits + passes - 1) / passes;  // balanced digits of at most 8 bits
+    const edpp_tuning saved = c->tune;
+    default_split_shape(c, 1 << per, 4, &c->tune.split_ipt, &c->tune.split_tile_mult);
+    int64_t hlen = 0;
+    for (int p = 0; p < passes; ++p) {
+        const int lo = begin_bit + p * per, hi = std::min(end_bit, lo + per);
+        hlen = std::max(hlen, his_len<uint32_t>(c, EDPP_SPLIT_WG, n, hi - lo));
+    }
+    // destinations from the last pass back: output, copy A, output, copy A, ...
+    // An in-place call whose first pass would write the output sends that pass
+    // to copy B instead; with a single pass it goes to copy A, which is then
+    // copied to the output.
+    const uint32_t* dk[8];
+    const int32_t* dv[8];
+    const bool alias = kin == kout || (vin && vin == vout);
+    const bool first_out = (passes % 2) == 1;  // the first pass writes the output
+    const bool need_a = passes > 1 || (alias && first_out);
+    const bool need_b = alias && first_out && passes > 1;
+    const size_t hbytes = ((size_t)hlen * 4 + 255) & ~(size_t)255;
+    const size_t kb = ((size_t)n * 4 + 255) & ~(size_t)255;
+    const size_t vb = vin ? kb : 0;
+    const size_t copies = (need_a ? 1 : 0) + (need_b ? 1 : 0);
+    void* w;
+    int rc = ws_reserve(c, hbytes + copies * (kb + vb), &w);
+    if (rc) {
+        c->tune = saved;
+        return rc;
+    }
+    char* base = static_cast<char*>(w);
+    int32_t* his = reinterpret_cast<int32_t*>(base);
+    uint32_t* ak = need_a ? reinterpret_cast<uint32_t*>(base + hbytes) : nullptr;
+    int32_t* av = (need_a && vin) ? reinterpret_cast<int32_t*>(base + hbytes + kb) : nullptr;
+    uint32_t* bk = need_b ? reinterpret_cast<uint32_t*>(base + hbytes + kb + vb) : nullptr;
+    int32_t* bv = (need_b && vin) ? reinterpret_cast<int32_t*>(base + hbytes + kb + vb + kb) : nullptr;
+    {
+        bool out = true;
+        for (int p = passes - 1; p >= 0; --p, out = !out) {
+            dk[p] = out ? kout : ak;
+            dv[p] = out ? vout : av;
+        }
+        if (alias && first_out) {
+            dk[0] = passes > 1 ? bk : ak;
+            dv[0] = passes > 1 ? bv : av;
+        }
+    }
+    const uint32_t* sk = kin;
+    const int32_t* sv = vin;
+    for (int p = 0; p < passes && rc == EDPP_OK; ++p) {
+        const int lo = begin_bit + p * per, hi = std::min(end_bit, lo + per), lg = hi - lo;
+        const uint32_t flip = (is_signed && hi == 32) ? (1u << (lg - 1)) : 0u;
+        SplitCall<uint32_t> a = {sk, const_cast<uint32_t*>(dk[p]), sv, const_cast<int32_t*>(dv[p]), nullptr, n, lg, lo,
+                                 flip};
+        rc = run_pass<uint32_t>(c, EDPP_SPLIT_WG, a, his);
+        sk = dk[p];
+        sv = dv[p];
+    }
+    c->tune = saved;
+    if (rc == EDPP_OK && sk != kout) {  // single in-place pass: copy A to the output
+        EDPP_HIP(c, hipMemcpyAsync(kout, sk, (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream));
+        if (vin) EDPP_HIP(c, hipMemcpyAsync(vout, sv, (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream));
+    }
+    return rc;
 }
 
 template int launch_split<uint32_t>(edpp_ctx*, const uint32_t*, uint32_t*, const int32_t*, int32_t*, int32_t*, int64_t,

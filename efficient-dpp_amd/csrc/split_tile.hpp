@@ -70,11 +70,20 @@ __host__ __device__ __forceinline__ int kbuf_word_offset(int B, int waves = kSpl
 struct SplitGeom {
     int64_t n;
     int lg;          // log2 B
-    int shift;       // bucket = (key >> shift) & (B-1)
+    int shift;       // bucket = ((key >> shift) & (B-1)) ^ flip
     int64_t tile;    // elements per tile (a multiple of the sub-tile)
     int64_t tiles;
     int xcds;
+    uint32_t flip;   // 0, or the top bucket bit: buckets of a signed top digit in order
 };
+
+// Bucket of a key: bits [shift, shift + log2 B) of its low 32 bits, with the
+// flip bits inverted (a radix sort's signed top digit orders negative keys
+// first when its top bit is flipped).
+template <class K>
+__device__ __forceinline__ uint32_t bucket_of(K key, int shift, uint32_t mask, uint32_t flip) {
+    return ((((uint32_t)key) >> shift) & mask) ^ flip;
+}
 
 // Loads the calling wave's chunk of a sub-tile (item j of lane l is element
 // sbase + w*64*IPT + j*64 + l), and its values when vin is given, then
@@ -84,7 +93,7 @@ struct SplitGeom {
 template <class K, int IPT>
 __device__ __forceinline__ void load_subtile(const K* __restrict__ kin, const int32_t* __restrict__ vin, int64_t sbase,
                                              int64_t n, int lg, int shift, uint32_t (&bk)[IPT], K (&key)[IPT],
-                                             int32_t (&val)[IPT]) {
+                                             int32_t (&val)[IPT], uint32_t flip = 0) {
     const int lane = lane_id();
     const int w = threadIdx.x / kWave;
     const uint32_t mask = (1u << lg) - 1u;
@@ -105,7 +114,7 @@ __device__ __forceinline__ void load_subtile(const K* __restrict__ kin, const in
     #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         const int64_t e = wbase + (int64_t)j * kWave + lane;
-        bk[j] = e < n ? (((uint32_t)key[j] >> shift) & mask) : 0xFFFFFFFFu;
+        bk[j] = e < n ? bucket_of(key[j], shift, mask, flip) : 0xFFFFFFFFu;
     }
 }
 
@@ -159,7 +168,7 @@ __host__ __device__ constexpr int reg_count_words(int lg) { return lg <= 2 ? 1 :
 // atomics.
 template <class K, int IPT, int TPW, int REGW>
 __device__ __forceinline__ void count_tiles(const K* __restrict__ kin, int64_t sbase, int64_t n, int lg, int shift,
-                                            int spt, uint32_t* whist) {
+                                            int spt, uint32_t* whist, uint32_t flip = 0) {
     constexpr int S = kSplitBlock * IPT;
     static_assert(IPT <= 255, "8-bit per-lane counters");
     const int lane = lane_id();
@@ -186,7 +195,7 @@ __device__ __forceinline__ void count_tiles(const K* __restrict__ kin, int64_t s
             #pragma unroll
             for (int j = 0; j < IPT; ++j) {
                 const int64_t e = wbase + (int64_t)t * S + (int64_t)j * kWave;
-                const uint32_t b = (((uint32_t)key[t][j]) >> shift) & mask;
+                const uint32_t b = bucket_of(key[t][j], shift, mask, flip);
                 const uint32_t inc = e < n ? 1u << (8 * (b & 3u)) : 0u;
                 #pragma unroll
                 for (int q = 0; q < REGW; ++q) packed[q] += (b >> 2) == (uint32_t)q ? inc : 0u;
@@ -209,16 +218,16 @@ __device__ __forceinline__ void count_tiles(const K* __restrict__ kin, int64_t s
             // bucket on the first or the middle item) take the grouped path
             // for the whole sub-tile; the test is wave-uniform
             uint32_t bf0, bf1;
-            const uint64_t mf0 = first_bucket_group((((uint32_t)key[t][0]) >> shift) & mask,
+            const uint64_t mf0 = first_bucket_group(bucket_of(key[t][0], shift, mask, flip),
                                                     wbase + (int64_t)t * S < n, &bf0);
-            const uint64_t mf1 = first_bucket_group((((uint32_t)key[t][IPT / 2]) >> shift) & mask,
+            const uint64_t mf1 = first_bucket_group(bucket_of(key[t][IPT / 2], shift, mask, flip),
                                                     wbase + (int64_t)t * S + (int64_t)(IPT / 2) * kWave < n, &bf1);
             if (__popcll(mf0) >= kWave / 8 || __popcll(mf1) >= kWave / 8) {
                 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
                     const int64_t e = wbase + (int64_t)t * S + (int64_t)j * kWave;
                     const bool ok = e < n;
-                    const uint32_t b = (((uint32_t)key[t][j]) >> shift) & mask;
+                    const uint32_t b = bucket_of(key[t][j], shift, mask, flip);
                     uint32_t bf;
                     const uint64_t mf = first_bucket_group(b, ok, &bf);
                     if (ok && b == bf) {
@@ -234,7 +243,7 @@ __device__ __forceinline__ void count_tiles(const K* __restrict__ kin, int64_t s
                 for (int j = 0; j < IPT; ++j) {
                     const int64_t e = wbase + (int64_t)t * S + (int64_t)j * kWave;
                     if (e < n)
-                        __hip_atomic_fetch_add(row + ((((uint32_t)key[t][j]) >> shift) & mask), 1u, __ATOMIC_RELAXED,
+                        __hip_atomic_fetch_add(row + bucket_of(key[t][j], shift, mask, flip), 1u, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_WAVEFRONT);
                 }
             }
@@ -258,9 +267,9 @@ template <class K, int IPT, bool BALLOT>
 __device__ __forceinline__ void rank_subtile(const K* __restrict__ kin, const int32_t* __restrict__ vin,
                                              int64_t sbase, int64_t n, int lg, int shift, uint32_t* whist_w,
                                              uint64_t* wmask, uint32_t (&bk)[IPT], uint32_t (&rk)[IPT],
-                                             K (&key)[IPT], int32_t (&val)[IPT]) {
+                                             K (&key)[IPT], int32_t (&val)[IPT], uint32_t flip = 0) {
     const int lane = lane_id();
-    load_subtile<K, IPT>(kin, vin, sbase, n, lg, shift, bk, key, val);
+    load_subtile<K, IPT>(kin, vin, sbase, n, lg, shift, bk, key, val, flip);
     uint64_t m[IPT];
     #pragma unroll
     for (int j = 0; j < IPT; ++j) {
@@ -339,7 +348,8 @@ __device__ __forceinline__ void place_and_store(uint32_t* whist, uint32_t* run, 
                                                 uint32_t* wtmp, K* kbuf, int32_t* vbuf, const uint32_t (&bk)[IPT],
                                                 const uint32_t (&rk)[IPT], const K (&key)[IPT],
                                                 const int32_t (&val)[IPT], int B, int shift, int cnt,
-                                                K* __restrict__ kout, int32_t* __restrict__ vout, int64_t n) {
+                                                K* __restrict__ kout, int32_t* __restrict__ vout, int64_t n,
+                                                uint32_t flip = 0) {
     constexpr int W = BLOCK / kWave;
     constexpr int S = BLOCK * IPT;
     const int w = threadIdx.x / kWave;
@@ -387,7 +397,7 @@ __device__ __forceinline__ void place_and_store(uint32_t* whist, uint32_t* run, 
         for (int j = 0; j < IPT; ++j) {
             const int i = threadIdx.x + j * BLOCK;
             const K k = kbuf[i < cnt ? i : 0];
-            d[j] = i < cnt ? dlt[((uint32_t)k >> shift) & mask] + (uint32_t)i : 0xFFFFFFFFu;
+            d[j] = i < cnt ? dlt[bucket_of(k, shift, mask, flip)] + (uint32_t)i : 0xFFFFFFFFu;
             if ((int64_t)d[j] < n) kout[d[j]] = k;  // false only for empty slots
         }
         __syncthreads();  // every key read before the values overwrite the buffer
@@ -403,7 +413,7 @@ __device__ __forceinline__ void place_and_store(uint32_t* whist, uint32_t* run, 
     } else {
         for (int i = threadIdx.x; i < cnt; i += BLOCK) {
             const K k = kbuf[i];
-            const uint32_t p = dlt[((uint32_t)k >> shift) & mask] + (uint32_t)i;
+            const uint32_t p = dlt[bucket_of(k, shift, mask, flip)] + (uint32_t)i;
             if ((int64_t)p < n) {  // always true for a consistent histogram scan
                 kout[p] = k;
                 if (KV) vout[p] = vbuf[i];

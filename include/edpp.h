@@ -20,6 +20,7 @@
  *   WI_split          opencl/primitives.h:32-37   -> edpp_split_i32 (EDPP_SPLIT_WI)
  *   WG_split          opencl/primitives.h:39-44   -> edpp_split_i32 (EDPP_SPLIT_WG*)
  *   single_split      opencl/primitives.h:46-49   -> edpp_split_i32 (EDPP_SPLIT_SINGLE)
+ *   radixSort         obsolete/CUDA/inc/kernels.h:37-45 -> edpp_sort_u32 / edpp_sort_i32
  *   STREAM kernels    opencl/kernels/mem_kernel.cl:9-31 -> edpp_stream_i32
  *   data generators   opencl/util/utility.cpp:214-241 -> edpp_gen_* (reproducible)
  *
@@ -169,6 +170,22 @@ int edpp_split_i32(edpp_ctx* ctx, const int32_t* keys_in, int32_t* keys_out,
 int edpp_split_i64(edpp_ctx* ctx, const int64_t* keys_in, int64_t* keys_out,
                    const int32_t* vals_in, int32_t* vals_out, int32_t* d_bucket_start,
                    int64_t n, int buckets, int strategy, double* ms);
+
+/* ---- LSD radix sort on the stable split -------------------------------------
+ * Replaces the reference's radixSort (obsolete/CUDA/kernels/radixSort.cu:29-200,
+ * declared at obsolete/CUDA/inc/kernels.h:37-45) and its CUB SortKeys /
+ * SortPairs calls (obsolete/CUDA/radixSortCUB.cu:25,93): stable sort of n
+ * 32-bit keys by key bits [begin_bit, end_bit) (0 <= begin <= end <= 32),
+ * values (both NULL or both non-NULL) following their keys.  _u32 orders keys
+ * as unsigned, _i32 as signed (bit 31 inverted when it is among the sorted
+ * bits).  keys_in may equal keys_out (and vals_in vals_out); otherwise the
+ * buffers must not overlap.                                                    */
+int edpp_sort_u32(edpp_ctx* ctx, const uint32_t* keys_in, uint32_t* keys_out,
+                  const int32_t* vals_in, int32_t* vals_out, int64_t n,
+                  int begin_bit, int end_bit, double* ms);
+int edpp_sort_i32(edpp_ctx* ctx, const int32_t* keys_in, int32_t* keys_out,
+                  const int32_t* vals_in, int32_t* vals_out, int64_t n,
+                  int begin_bit, int end_bit, double* ms);
 
 /* ---- reproducible synthetic inputs ---------------------------------------
  * Replace opencl/util/utility.cpp:214-241 (time-seeded, schedule-dependent)

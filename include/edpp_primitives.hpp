@@ -65,4 +65,11 @@ double WG_split(const int* d_in, int* d_out, int* d_start, int length, int bucke
 // opencl/primitives.h:46-49 (KO and KVS_AOS only, as in the reference)
 double single_split(const int* d_in, int* d_out, int length, int buckets, bool reorder, DataStruc structure);
 
+// obsolete/CUDA/inc/kernels.h:37-45 (the reference's radixSort): sorts in
+// place, ascending as int32 like its std::sort check
+// (obsolete/CUDA/src/testRadixSort.cpp:57-60).  isRecord: d_source_keys are
+// the keys and d_source_values follow them; otherwise d_source_values are the
+// keys and d_source_keys is unused.
+double radixSort(int* d_source_keys, int* d_source_values, int len, bool isRecord);
+
 }  // namespace edpp

@@ -248,6 +248,53 @@ void oracle_split(const void* keys_in, void* keys_out, const int32_t* vals_in, i
     free(count);
 }
 
+/* ------------------------------------------------------------------ radix sort
+ * LSD radix sort as the reference's radixSort runs it
+ * (obsolete/CUDA/kernels/radixSort.cu:20-26: count the digits, exclusive-scan
+ * the counts, scatter; SORT_BITS = 4 digits per pass,
+ * obsolete/CUDA/inc/dataDef.h:16-17), one digit of key bits [begin, end) per
+ * pass, lowest first; every pass is a stable counting sort, so the result is
+ * the stable sort by those bits.  is_signed orders the keys as int32 (bit 31
+ * inverted when it is among the sorted bits).  vals may be NULL.             */
+void oracle_sort_u32(const uint32_t* keys, const int32_t* vals, uint32_t* keys_out, int32_t* vals_out, int64_t n,
+                     int begin_bit, int end_bit, int is_signed) {
+    uint32_t* ka = (uint32_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(uint32_t));
+    int32_t* va = (int32_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    uint32_t* kb = (uint32_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(uint32_t));
+    int32_t* vb = (int32_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    const uint32_t twiddle = is_signed ? 0x80000000u : 0u;
+    for (int64_t i = 0; i < n; ++i) {
+        ka[i] = keys[i] ^ twiddle;
+        va[i] = vals ? vals[i] : 0;
+    }
+    for (int lo = begin_bit; lo < end_bit; lo += 4) {
+        const int bits = end_bit - lo < 4 ? end_bit - lo : 4;
+        const uint32_t mask = (1u << bits) - 1u;
+        int64_t count[16] = {0}, pos[16];
+        for (int64_t i = 0; i < n; ++i) count[(ka[i] >> lo) & mask]++;
+        int64_t acc = 0;
+        for (int d = 0; d < 16; ++d) {
+            pos[d] = acc;
+            acc += count[d];
+        }
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t p = pos[(ka[i] >> lo) & mask]++;
+            kb[p] = ka[i];
+            vb[p] = va[i];
+        }
+        uint32_t* tk = ka; ka = kb; kb = tk;
+        int32_t* tv = va; va = vb; vb = tv;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        keys_out[i] = ka[i] ^ twiddle;
+        if (vals_out) vals_out[i] = va[i];
+    }
+    free(ka);
+    free(va);
+    free(kb);
+    free(vb);
+}
+
 /* ------------------------------------------------------------------ STREAM
  * opencl/kernels/mem_kernel.cl:9-31 with SCALAR = 3 (mem_kernel.cl:6):
  * 0 copy b=a, 1 scale b=3a, 2 add c=a+b, 3 triad c=a+3b (int32 wrap)       */

@@ -43,6 +43,7 @@ def oracle():
             "oracle_gather": (None, [_P, _P, _P, _I64, ctypes.c_int, ctypes.c_int]),
             "oracle_scatter": (None, [_P, _P, _P, _I64, ctypes.c_int, ctypes.c_int]),
             "oracle_split": (None, [_P, _P, _P, _P, _P, _I64, ctypes.c_int, ctypes.c_int]),
+            "oracle_sort_u32": (None, [_P, _P, _P, _P, _I64, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             "oracle_stream": (None, [ctypes.c_int, _P, _P, _P, _I64]),
             "oracle_hampel_mean": (ctypes.c_double, [_P, ctypes.c_int]),
         }
@@ -175,6 +176,18 @@ def split(keys, B, vals=None):
     oracle().oracle_split(_a(keys), _a(ko), None if vals is None else _a(np.ascontiguousarray(vals)),
                           None if vo is None else _a(vo), _a(start), keys.size, B, keys.dtype.itemsize)
     return ko, vo, start
+
+
+def sort(keys, vals=None, begin_bit=0, end_bit=32, signed=False):
+    """Stable LSD radix sort of 32-bit keys by bits [begin_bit, end_bit);
+    returns (keys_out, vals_out or None)."""
+    keys = np.ascontiguousarray(keys)
+    assert keys.dtype.itemsize == 4
+    ko = np.empty_like(keys)
+    vo = None if vals is None else np.empty_like(vals)
+    oracle().oracle_sort_u32(_a(keys), None if vals is None else _a(np.ascontiguousarray(vals)), _a(ko),
+                             None if vo is None else _a(vo), keys.size, begin_bit, end_bit, int(signed))
+    return ko, vo
 
 
 def stream(op, a, b=None):

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <algorithm>
 #include <vector>
 #include "edpp_primitives.hpp"
 
@@ -178,6 +179,37 @@ static void test_splits(int len, int buckets) {
     hipFree(d_start);
 }
 
+// radixSort (obsolete/CUDA/inc/kernels.h:37-45) checked against std::sort /
+// std::stable_sort, as the reference's own test does
+static void test_radix_sort(int len) {
+    std::vector<int> keys(len), vals(len);
+    unsigned s = 12345u;
+    for (int i = 0; i < len; ++i) {
+        s = s * 1664525u + 1013904223u;
+        keys[i] = (int)s;
+        if (i % 7 == 0 && i > 0) keys[i] = keys[i / 2];  // duplicates
+        vals[i] = i;
+    }
+    int* d_keys = dev(keys);
+    int* d_vals = dev(vals);
+    double ms = edpp::radixSort(d_keys, d_vals, len, true);
+    std::vector<int> ko = host(d_keys, len), vo = host(d_vals, len);
+    std::vector<int> order(len);
+    for (int i = 0; i < len; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return keys[a] < keys[b]; });
+    bool ok = ms >= 0;
+    for (int i = 0; i < len && ok; ++i) ok = vo[i] == order[i] && ko[i] == keys[order[i]];
+    CHECK(ok, "radixSort records len %d", len);
+    // keys only: d_source_values are the keys
+    hipMemcpy(d_vals, keys.data(), sizeof(int) * (size_t)len, hipMemcpyHostToDevice);
+    ms = edpp::radixSort(nullptr, d_vals, len, false);
+    std::vector<int> sorted = keys;
+    std::sort(sorted.begin(), sorted.end());
+    CHECK(ms >= 0 && host(d_vals, len) == sorted, "radixSort keys len %d", len);
+    hipFree(d_keys);
+    hipFree(d_vals);
+}
+
 int main(int argc, char** argv) {
     int len = argc > 1 ? atoi(argv[1]) : (1 << 20);
     if (!edpp::context()) {
@@ -191,6 +223,8 @@ int main(int argc, char** argv) {
     test_matrix_scans();
     for (int b = 2; b <= 4096; b *= 8) test_splits(len, b);
     test_splits(len + 13, 4096);
+    test_radix_sort(len);
+    test_radix_sort(len / 3 + 1);
     printf("%s: %d failure(s)\n", failures ? "FAILED" : "OK", failures);
     return failures ? 1 : 0;
 }

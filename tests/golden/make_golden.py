@@ -2,7 +2,8 @@
 outputs of the sequential oracle (oracle/oracle.c), each output also checked
 here against an independent closed form before it is written (numpy cumsum
 for the scan, fancy indexing for gather/scatter, a stable argsort and
-bincount for the split, element-wise formulas for STREAM).  Run from the
+bincount for the split, a stable argsort of the key bits for the radix sort,
+element-wise formulas for STREAM).  Run from the
 repository root after ``python oracle/build.py``:
 
     python tests/golden/make_golden.py
@@ -97,6 +98,25 @@ def _split():
     return out
 
 
+def _sort():
+    out = {}
+    for seed in SEEDS:
+        for n in (1, 100, 4096):
+            for signed in (0, 1):
+                for lo, hi in ((0, 32), (4, 20)):
+                    k = orc.gen_i32(n, seed + n + lo)
+                    k[: n // 8] = k[n // 8: 2 * (n // 8)]  # duplicates
+                    k = k if signed else k.view(np.uint32)
+                    v = np.arange(n, dtype=np.int32)
+                    ko, vo = orc.sort(k, v, lo, hi, bool(signed))
+                    u = k.view(np.uint32).astype(np.uint64) ^ np.uint64(0x80000000 if signed else 0)
+                    order = np.argsort((u >> np.uint64(lo)) & np.uint64((1 << (hi - lo)) - 1), kind="stable")
+                    assert np.array_equal(vo, order) and np.array_equal(ko, k[order])
+                    key = f"s{seed}_n{n}_g{signed}_b{lo}_{hi}"
+                    out[key + "_keys"], out[key + "_kout"], out[key + "_vout"] = k.view(np.int32), ko.view(np.int32), vo
+    return out
+
+
 def _stream():
     out = {}
     for n in (1, 5, 1027):
@@ -130,7 +150,7 @@ def _generators():
 
 def main():
     for name, fn in (("scan", _scan), ("gather_scatter", _gather_scatter), ("split", _split),
-                     ("stream", _stream), ("generators", _generators)):
+                     ("sort", _sort), ("stream", _stream), ("generators", _generators)):
         path = os.path.join(HERE, f"{name}.npz")
         np.savez_compressed(path, **fn())
         print(path, os.path.getsize(path))

@@ -75,6 +75,25 @@ def test_split_fixtures(orc):
         assert np.array_equal(ko2, ko) and np.array_equal(vo2, vo) and np.array_equal(st2, st), k
 
 
+def test_sort_fixtures(orc):
+    z = load("sort")
+    keys = cases(z, "_keys")
+    assert len(keys) >= 12
+    for k in keys:
+        signed = "_g1_" in k
+        lo, hi = (int(x) for x in k.split("_b")[1].split("_"))
+        kin, ko, vo = z[k + "_keys"], z[k + "_kout"], z[k + "_vout"]
+        u = kin.view(np.uint32).astype(np.uint64) ^ np.uint64(0x80000000 if signed else 0)
+        d = (u >> np.uint64(lo)) & np.uint64((1 << (hi - lo)) - 1)
+        assert np.array_equal(np.sort(vo), np.arange(len(kin))), k          # a permutation
+        assert np.array_equal(ko, kin[vo]), k                                # keys follow values
+        dd = d[vo]
+        assert np.all(dd[1:] >= dd[:-1]), k                                  # sorted by the key bits
+        assert np.all((dd[1:] > dd[:-1]) | (vo[1:] > vo[:-1])), k           # ties in input order
+        k2, v2 = orc.sort(kin if signed else kin.view(np.uint32), np.arange(len(kin), dtype=np.int32), lo, hi, signed)
+        assert np.array_equal(k2.view(np.int32), ko) and np.array_equal(v2, vo), k
+
+
 def test_stream_fixtures(orc):
     z = load("stream")
     for n in (1, 5, 1027):
@@ -106,7 +125,7 @@ def test_generator_fixtures(orc):
                     v, z[f"s{seed}_vals_B{B}_d{dist}"])
 
 
-@pytest.mark.parametrize("name", ["scan", "gather_scatter", "split", "stream", "generators"])
+@pytest.mark.parametrize("name", ["scan", "gather_scatter", "split", "sort", "stream", "generators"])
 def test_fixtures_are_plain_arrays(name):
     z = load(name)
     for k in z.files:

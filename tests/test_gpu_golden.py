@@ -75,6 +75,22 @@ def test_golden_split(torch_gpu, ctx):
             assert np.array_equal(host(st), z[k + "_start"]), (k, strategy)
 
 
+def test_golden_sort(torch_gpu, ctx):
+    torch = torch_gpu
+    z = load("sort")
+    for k in cases(z, "_keys"):
+        signed = "_g1_" in k
+        lo, hi = (int(x) for x in k.split("_b")[1].split("_"))
+        kin = z[k + "_keys"]
+        n = len(kin)
+        kout = torch.empty(n, dtype=torch.int32, device="cuda")
+        vout = torch.empty(n, dtype=torch.int32, device="cuda")
+        vals = torch.arange(n, dtype=torch.int32, device="cuda")
+        ctx.sort(dev(torch, kin), kout, vals, vout, n, lo, hi, signed)
+        assert np.array_equal(host(kout), z[k + "_kout"]), k
+        assert np.array_equal(host(vout), z[k + "_vout"]), k
+
+
 def test_golden_stream(torch_gpu, ctx):
     torch = torch_gpu
     z = load("stream")

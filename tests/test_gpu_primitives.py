@@ -326,3 +326,49 @@ def test_split_full_size_properties(torch_gpu, ctx, B):
     exp = torch.cumsum(cnt, 0) - cnt
     assert torch.equal(st.long(), exp)
     del keys, vals, kout, vout, b, same
+
+
+# ---------------------------------------------------------------- radix sort
+@pytest.mark.parametrize("signed", [False, True])
+@pytest.mark.parametrize("bits", [(0, 32), (0, 16), (4, 20), (8, 32), (0, 8), (3, 4), (7, 7)])
+@pytest.mark.parametrize("n", [0, 1, 1000, 100_003, (1 << 20) + 17])
+def test_sort(torch_gpu, ctx, orc, signed, bits, n):
+    torch = torch_gpu
+    keys = orc.gen_i32(n, 500 + n + bits[0])
+    if n > 10:
+        keys[: n // 10] = keys[n // 10: n // 5]  # duplicates: stability observable
+    keys = keys if signed else keys.view(np.uint32)
+    vals = np.arange(n, dtype=np.int32)
+    ko, vo = orc.sort(keys, vals, bits[0], bits[1], signed)
+    dk = dev(torch, keys.view(np.int32))
+    kout = empty(torch, n, torch.int32)
+    vout = empty(torch, n, torch.int32)
+    ctx.sort(dk, kout, dev(torch, vals), vout, n, bits[0], bits[1], signed)
+    assert np.array_equal(host(kout), ko.view(np.int32)), (signed, bits, n)
+    assert np.array_equal(host(vout), vo), (signed, bits, n)
+    k2 = empty(torch, n, torch.int32)
+    ctx.sort(dk, k2, n=n, begin_bit=bits[0], end_bit=bits[1], signed=signed)  # keys only
+    assert np.array_equal(host(k2), ko.view(np.int32))
+    # in place (keys and values)
+    vin = dev(torch, vals)
+    ctx.sort(dk, dk, vin, vin, n, bits[0], bits[1], signed)
+    assert np.array_equal(host(dk), ko.view(np.int32)) and np.array_equal(host(vin), vo), ("in place", bits, n)
+
+
+def test_sort_full_size_properties(torch_gpu, ctx):
+    # 2^28 pairs, signed 32-bit keys, vals = original index: keys ascending,
+    # ties in input order, a permutation, keys follow their values
+    torch = torch_gpu
+    n = 1 << 28
+    keys = empty(torch, n, torch.int32)
+    ctx.gen_i32(keys, 0x50F7)
+    vals = torch.arange(n, dtype=torch.int32, device="cuda")
+    kout = empty(torch, n, torch.int32)
+    vout = empty(torch, n, torch.int32)
+    ctx.sort(keys, kout, vals, vout, n, 0, 32, True)
+    assert bool((kout[1:] >= kout[:-1]).all())
+    same = kout[1:] == kout[:-1]
+    assert bool((vout[1:][same] > vout[:-1][same]).all())
+    assert torch.equal(keys[vout.long()], kout)
+    assert int(torch.bincount(vout.long(), minlength=n).max()) == 1
+    del keys, vals, kout, vout, same

@@ -259,3 +259,46 @@ def test_hampel_matches_reference(orc):
         # the reference sums kept samples in order of deviation, the oracle in
         # input order: equal up to rounding of the final sum
         assert orc.hampel_mean(x) == pytest.approx(ref.edpp_ref_average_hampel(orc._a(x.copy()), x.size), rel=1e-12)
+
+
+# ---------------------------------------------------------------- radix sort
+def _twiddled_bits(k, begin, end, signed):
+    u = k.view(np.uint32).astype(np.uint64)
+    if signed:
+        u ^= np.uint64(0x80000000)
+    return (u >> np.uint64(begin)) & np.uint64((1 << (end - begin)) - 1)
+
+
+@pytest.mark.parametrize("signed", [False, True])
+@pytest.mark.parametrize("bits", [(0, 32), (0, 8), (4, 20), (8, 32), (31, 32), (5, 5)])
+def test_sort_vs_numpy_stable_argsort(orc, signed, bits):
+    rng = np.random.default_rng(7 + bits[0] + 40 * bits[1])
+    n = 20_000
+    keys = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    keys[: n // 10] = keys[n // 10: n // 5]  # duplicates make stability observable
+    if signed:
+        keys = keys.view(np.int32)
+    vals = np.arange(n, dtype=np.int32)
+    ko, vo = orc.sort(keys, vals, bits[0], bits[1], signed)
+    order = np.argsort(_twiddled_bits(keys, bits[0], bits[1], signed), kind="stable")
+    assert np.array_equal(vo, order)
+    assert np.array_equal(ko, keys[order])
+
+
+def test_sort_full_key_orders_like_std_sort(orc):
+    # the reference checks its radix sort against std::sort of the keys
+    # (obsolete/CUDA/src/testRadixSort.cpp:57-60)
+    rng = np.random.default_rng(3)
+    k = rng.integers(-2**31, 2**31, 50_000, dtype=np.int64).astype(np.int32)
+    assert np.array_equal(orc.sort(k, signed=True)[0], np.sort(k))
+    assert np.array_equal(orc.sort(k.view(np.uint32))[0], np.sort(k.view(np.uint32)))
+
+
+def test_sort_brute_force_tiny(orc):
+    rng = np.random.default_rng(11)
+    for n in range(0, 9):
+        k = rng.integers(0, 8, n).astype(np.uint32)
+        v = np.arange(n, dtype=np.int32)
+        ko, vo = orc.sort(k, v, 0, 3)
+        ref = sorted(range(n), key=lambda i: (int(k[i]), i))  # stable by construction
+        assert list(vo) == ref and list(ko) == [int(k[i]) for i in ref]
