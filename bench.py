@@ -405,6 +405,19 @@ def suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs):
                      "gbs_one_pass_bytes": round(gbs(per * n, t["hampel_ms"]), 1)}
     res["sort"] = srt
     print("sort", srt, file=sys.stderr, flush=True)
+    # partitioned hash join count: two relations of 2^24 and of 2^26 distinct keys
+    jn = {}
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    for lg in (24, 26):
+        m = 1 << lg
+        ctx.gen_perm(ko[:m], 0x1A, edpp.PERM_RANDOM, n=m)
+        ctx.gen_perm(vo[:m], 0x1B, edpp.PERM_RANDOM, n=m)
+        t = T.time(lambda: ctx.join_count(ko[:m], vo[:m], cnt, m, m), reps=5)
+        assert int(cnt.item()) == m, "join count mismatch"
+        jn[f"2^{lg}x2^{lg}"] = {"ms": round(t["hampel_ms"], 3),
+                                "g_tuples_s": round(2 * m / (t["hampel_ms"] * 1e-3) / 1e9, 2)}
+    res["hash_join"] = jn
+    print("hash_join", jn, file=sys.stderr, flush=True)
     del ko, vo, b
     ctx.gen_i32(src, 0xDA7A)
     return res

@@ -45,6 +45,7 @@ _SIGS = {
     "edpp_gather_i64": [c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_dp],
     "edpp_scatter_i64": [c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_dp],
     "edpp_default_passes": [c_vp, c_i64, c_int, c_int, ctypes.POINTER(c_int)],
+    "edpp_hash_join_count": [c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_dp],
     "edpp_sort_u32": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_dp],
     "edpp_sort_i32": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_dp],
     "edpp_scan_exclusive_i32": [c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_dp],
@@ -268,6 +269,12 @@ class Context:
         name = "edpp_sort_i32" if signed else "edpp_sort_u32"
         return self._call(name, _ptr(keys_in), _ptr(keys_out), _ptr(vals_in), _ptr(vals_out), n, int(begin_bit),
                           int(end_bit), timed=timed)
+
+    def join_count(self, r_keys, s_keys, count, r_n=None, s_n=None, timed=False):
+        """count[0] (uint64/int64 device tensor) = number of equal-key pairs of R and S."""
+        r_n = _numel(r_keys) if r_n is None else r_n
+        s_n = _numel(s_keys) if s_n is None else s_n
+        return self._call("edpp_hash_join_count", _ptr(r_keys), r_n, _ptr(s_keys), s_n, _ptr(count), timed=timed)
 
     def schedule_passes(self, n, scatter=False, wide=False):
         """Pass count a passes=0 gather/scatter of n elements uses without a cost model."""

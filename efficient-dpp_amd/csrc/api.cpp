@@ -29,24 +29,28 @@ int hip_fail(edpp_ctx* c, hipError_t e, const char* what) {
     return fail(c, EDPP_EHIP, "%s: %s", what, hipGetErrorString(e));
 }
 
-int ws_reserve(edpp_ctx* c, size_t bytes, void** out) {
+static int grow(edpp_ctx* c, void** ws, size_t* ws_bytes, size_t bytes, void** out) {
     bytes = (bytes + 255) & ~(size_t)255;
-    if (bytes > c->ws_bytes) {
-        if (c->ws) {
+    if (bytes > *ws_bytes) {
+        if (*ws) {
             EDPP_HIP(c, hipStreamSynchronize(c->stream));  // earlier kernels may still use it
-            EDPP_HIP(c, hipFree(c->ws));
-            c->ws = nullptr;
-            c->ws_bytes = 0;
+            EDPP_HIP(c, hipFree(*ws));
+            *ws = nullptr;
+            *ws_bytes = 0;
         }
         size_t want = std::max(bytes, (size_t)1 << 20);
-        if (hipMalloc(&c->ws, want) != hipSuccess) {
-            c->ws = nullptr;
+        if (hipMalloc(ws, want) != hipSuccess) {
+            *ws = nullptr;
             return fail(c, EDPP_ENOMEM, "workspace of %zu bytes", want);
         }
-        c->ws_bytes = want;
+        *ws_bytes = want;
     }
-    *out = c->ws;
+    *out = *ws;
     return EDPP_OK;
+}
+int ws_reserve(edpp_ctx* c, size_t bytes, void** out) { return grow(c, &c->ws, &c->ws_bytes, bytes, out); }
+int ws_outer_reserve(edpp_ctx* c, size_t bytes, void** out) {
+    return grow(c, &c->ws_outer, &c->ws_outer_bytes, bytes, out);
 }
 
 int scan_state_acquire(edpp_ctx* c, int64_t tiles, uint32_t* epoch_out) {
@@ -223,6 +227,7 @@ int edpp_ctx_destroy(edpp_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     hipFree(c->ws);
+    hipFree(c->ws_outer);
     hipFree(c->scan_status);
     hipFree(c->scan_status64);
     hipFree(c->tickets);
@@ -478,6 +483,21 @@ int edpp_sort_i32(edpp_ctx* c, const int32_t* keys_in, int32_t* keys_out, const 
                   int64_t n, int begin_bit, int end_bit, double* ms) {
     return sort_common(c, (const uint32_t*)keys_in, (uint32_t*)keys_out, vals_in, vals_out, n, begin_bit, end_bit, true,
                        ms);
+}
+
+// ---------------------------------------------------------------- hash join
+int edpp_hash_join_count(edpp_ctx* c, const int32_t* r_keys, int64_t r_n, const int32_t* s_keys, int64_t s_n,
+                         uint64_t* d_count, double* ms) {
+    EDPP_CHECK_CTX(c);
+    if (r_n < 0 || s_n < 0 || r_n >= (1ll << 31) || s_n >= (1ll << 31))
+        return fail(c, EDPP_EINVAL, "relation sizes outside [0, 2^31)");
+    if (!d_count || (r_n > 0 && !r_keys) || (s_n > 0 && !s_keys)) return fail(c, EDPP_EINVAL, "null buffer");
+    int rc = timing_begin(c, ms);
+    if (!rc)
+        rc = launch_join_count(c, (const uint32_t*)r_keys, r_n, (const uint32_t*)s_keys, s_n,
+                               (unsigned long long*)d_count);
+    if (!rc) rc = timing_end(c, ms);
+    return rc;
 }
 
 // ---------------------------------------------------------------- generators

@@ -18,6 +18,10 @@ struct edpp_ctx {
     // grow-only scratch for per-call temporaries (histograms, partials)
     void* ws = nullptr;
     size_t ws_bytes = 0;
+    // scratch of composite calls (hash join) that hold buffers across the
+    // primitives they call, which use ws
+    void* ws_outer = nullptr;
+    size_t ws_outer_bytes = 0;
 
     // persistent look-back state of the single-pass scans: tile status words
     // tagged with a per-call epoch, plus two ticket counters used alternately
@@ -57,6 +61,7 @@ int hip_fail(edpp_ctx* c, hipError_t e, const char* what);
 
 // workspace of at least `bytes` (256-byte aligned); may synchronise to grow
 int ws_reserve(edpp_ctx* c, size_t bytes, void** out);
+int ws_outer_reserve(edpp_ctx* c, size_t bytes, void** out);
 // look-back state for at least `tiles` tiles; returns the epoch for this call
 int scan_state_acquire(edpp_ctx* c, int64_t tiles, uint32_t* epoch_out);
 
@@ -82,6 +87,8 @@ int launch_split(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t
                  int32_t* d_start, int64_t n, int buckets, int strategy);
 int launch_sort_u32(edpp_ctx* c, const uint32_t* kin, uint32_t* kout, const int32_t* vin, int32_t* vout, int64_t n,
                     int begin_bit, int end_bit, bool is_signed);
+int launch_join_count(edpp_ctx* c, const uint32_t* rk, int64_t rn, const uint32_t* sk, int64_t sn,
+                      unsigned long long* d_count);
 int launch_gen_u32(edpp_ctx* c, uint32_t* out, int64_t n, uint64_t seed, uint32_t mask);
 int launch_gen_u64(edpp_ctx* c, uint64_t* out, int64_t n, uint64_t seed, uint64_t mask);
 int launch_gen_perm(edpp_ctx* c, int32_t* idx, int64_t n, uint64_t seed, int kind);

@@ -1,0 +1,166 @@
+// join.hip — partitioned hash join (count of equal-key pairs) on the stable
+// split.
+//
+// Reference: obsolete/OpenCL/phjImpl.cpp:14-66 (partition R and S with
+// WG_split into 2^13 buckets, then one workgroup per bucket pair builds an LDS
+// hash table of one side and probes it with the other,
+// obsolete/OpenCL/hj_partitioned_kernel.cl:7-115) and its count-only result
+// (phjImpl.cpp:13 "only count the number of outputs").  The count is
+//   sum over keys k of |{i: R[i] == k}| * |{j: S[j] == k}|.
+// Here the keys are first replaced by a bijective mix of their bits, so the
+// partitions stay balanced for structured keys; equal keys keep equal mixes.
+// The partition width targets 4096 build keys; a workgroup builds its table
+// from the smaller side of its pair in chunks of at most kJoinChunk keys
+// (load factor <= 1/2), so any skew ends, at the cost of re-probing.  Every
+// key is valid (no sentinel value, unlike the reference's -1).
+#include <algorithm>
+#include "ctx.hpp"
+#include "device.hpp"
+
+namespace edpp {
+
+namespace {
+
+constexpr int kJoinBlock = 1024;
+constexpr int kJoinSlots = 16384;  // 64-bit LDS slots (128 KiB): (count << 32) | key; count 0 = empty
+constexpr int kJoinChunk = 8192;   // build keys per table fill
+constexpr int kJoinTargetBuild = 4096;
+
+// murmur3's 32-bit finalizer: a bijection of 32-bit words
+__host__ __device__ inline uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x85EBCA6Bu;
+    x ^= x >> 13;
+    x *= 0xC2B2AE35u;
+    x ^= x >> 16;
+    return x;
+}
+
+__global__ void k_join_mix(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = mix32(in[i]);
+}
+
+__global__ void k_join_zero(unsigned long long* p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0ull;
+}
+
+// Slots are probed from the mixed key's bits above the partition bits.
+__device__ __forceinline__ uint32_t join_slot(uint32_t hk, int pbits) {
+    return (pbits >= 32 ? 0u : (hk >> pbits)) & (uint32_t)(kJoinSlots - 1);
+}
+
+__device__ __forceinline__ void join_insert(uint64_t* table, uint32_t hk, int pbits) {
+    uint32_t h = join_slot(hk, pbits);
+    uint64_t w = __hip_atomic_load(table + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    while (true) {
+        if ((w >> 32) == 0) {  // empty: claim it for this key with count 1
+            uint64_t expect = 0;
+            if (__hip_atomic_compare_exchange_strong(table + h, &expect, (1ull << 32) | hk, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+                return;
+            w = expect;  // claimed meanwhile: look at it again
+            continue;
+        }
+        if ((uint32_t)w == hk) {
+            __hip_atomic_fetch_add(table + h, 1ull << 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            return;
+        }
+        h = (h + 1) & (uint32_t)(kJoinSlots - 1);
+        w = __hip_atomic_load(table + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+__device__ __forceinline__ uint32_t join_lookup(const uint64_t* table, uint32_t hk, int pbits) {
+    uint32_t h = join_slot(hk, pbits);
+    while (true) {
+        const uint64_t w = table[h];
+        if ((w >> 32) == 0) return 0;
+        if ((uint32_t)w == hk) return (uint32_t)(w >> 32);
+        h = (h + 1) & (uint32_t)(kJoinSlots - 1);
+    }
+}
+
+// One workgroup per partition pair b: R rows [rs[b], rs[b+1]) and S rows
+// [ss[b], ss[b+1]) of the partitioned mixed keys.  A table fill holds at most
+// kJoinChunk distinct keys in kJoinSlots slots, so every probe sequence meets
+// an empty slot and ends.
+__global__ __launch_bounds__(kJoinBlock) void k_join_probe(const uint32_t* __restrict__ rk, int64_t rn,
+                                                           const int32_t* __restrict__ rs,
+                                                           const uint32_t* __restrict__ sk, int64_t sn,
+                                                           const int32_t* __restrict__ ss, int B, int pbits,
+                                                           unsigned long long* d_count) {
+    extern __shared__ uint64_t table[];  // kJoinSlots
+    const int b = blockIdx.x;
+    const int64_t r0 = rs[b], r1 = b + 1 < B ? rs[b + 1] : rn;
+    const int64_t s0 = ss[b], s1 = b + 1 < B ? ss[b + 1] : sn;
+    const bool build_r = (r1 - r0) <= (s1 - s0);
+    const uint32_t* bkeys = build_r ? rk : sk;
+    const uint32_t* pkeys = build_r ? sk : rk;
+    const int64_t b0 = build_r ? r0 : s0, b1 = build_r ? r1 : s1;
+    const int64_t p0 = build_r ? s0 : r0, p1 = build_r ? s1 : r1;
+    unsigned long long local = 0;
+    if (b1 > b0 && p1 > p0) {
+        for (int64_t c0 = b0; c0 < b1; c0 += kJoinChunk) {
+            const int64_t c1 = std::min<int64_t>(c0 + kJoinChunk, b1);
+            for (int i = threadIdx.x; i < kJoinSlots; i += kJoinBlock) table[i] = 0ull;
+            __syncthreads();
+            for (int64_t i = c0 + threadIdx.x; i < c1; i += kJoinBlock) join_insert(table, bkeys[i], pbits);
+            __syncthreads();
+            for (int64_t i = p0 + threadIdx.x; i < p1; i += kJoinBlock) local += join_lookup(table, pkeys[i], pbits);
+            __syncthreads();
+        }
+    }
+    // workgroup sum: wave sums, then one lane per wave into the table's first slots
+    const uint64_t ws = wave_sum((uint64_t)local);
+    __syncthreads();
+    if (lane_id() == 0) table[threadIdx.x / kWave] = ws;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long total = 0;
+        for (int q = 0; q < kJoinBlock / kWave; ++q) total += table[q];
+        if (total) atomicAdd(d_count, total);
+    }
+}
+
+}  // namespace
+
+int launch_join_count(edpp_ctx* c, const uint32_t* rk, int64_t rn, const uint32_t* sk, int64_t sn,
+                      unsigned long long* d_count) {
+    hipLaunchKernelGGL(k_join_zero, dim3(1), dim3(64), 0, c->stream, d_count);
+    EDPP_HIP(c, hipGetLastError());
+    if (rn == 0 || sn == 0) return EDPP_OK;
+    // partition bits: about kJoinTargetBuild keys of the smaller side per partition
+    const int64_t small = std::min(rn, sn);
+    int pbits = 0;
+    while (pbits < 12 && (small >> pbits) > kJoinTargetBuild) ++pbits;
+    const int B = 1 << pbits;
+    const size_t rb = ((size_t)rn * 4 + 255) & ~(size_t)255, sb = ((size_t)sn * 4 + 255) & ~(size_t)255;
+    const size_t stb = ((size_t)B * 4 + 255) & ~(size_t)255;
+    void* w;
+    int rc = ws_outer_reserve(c, 2 * rb + 2 * sb + 2 * stb, &w);
+    if (rc) return rc;
+    char* base = static_cast<char*>(w);
+    uint32_t* rmix = reinterpret_cast<uint32_t*>(base);
+    uint32_t* rpart = reinterpret_cast<uint32_t*>(base + rb);
+    uint32_t* smix = reinterpret_cast<uint32_t*>(base + 2 * rb);
+    uint32_t* spart = reinterpret_cast<uint32_t*>(base + 2 * rb + sb);
+    int32_t* rst = reinterpret_cast<int32_t*>(base + 2 * rb + 2 * sb);
+    int32_t* sst = reinterpret_cast<int32_t*>(base + 2 * rb + 2 * sb + stb);
+    const unsigned grid_r = (unsigned)std::min<int64_t>(ceil_div(rn, 256), (int64_t)c->cus * 8);
+    const unsigned grid_s = (unsigned)std::min<int64_t>(ceil_div(sn, 256), (int64_t)c->cus * 8);
+    hipLaunchKernelGGL(k_join_mix, dim3(grid_r), dim3(256), 0, c->stream, rk, rmix, rn);
+    hipLaunchKernelGGL(k_join_mix, dim3(grid_s), dim3(256), 0, c->stream, sk, smix, sn);
+    EDPP_HIP(c, hipGetLastError());
+    rc = launch_split<uint32_t>(c, rmix, rpart, nullptr, nullptr, rst, rn, B, EDPP_SPLIT_AUTO);
+    if (!rc) rc = launch_split<uint32_t>(c, smix, spart, nullptr, nullptr, sst, sn, B, EDPP_SPLIT_AUTO);
+    if (rc) return rc;
+    const size_t lds = (size_t)kJoinSlots * 8;
+    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_join_probe, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_join_probe, dim3((unsigned)B), dim3(kJoinBlock), lds, c->stream, rpart, rn, rst, spart, sn,
+                       sst, B, pbits, d_count);
+    EDPP_HIP(c, hipGetLastError());
+    return EDPP_OK;
+}
+
+}  // namespace edpp

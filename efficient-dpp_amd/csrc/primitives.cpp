@@ -2,6 +2,7 @@
 // over the C ABI.  Every function validates its arguments the way the
 // reference does, runs synchronously on the process-wide context and returns
 // device milliseconds.
+#include <hip/hip_runtime.h>
 #include <mutex>
 #include "edpp_primitives.hpp"
 
@@ -126,6 +127,26 @@ double radixSort(int* d_source_keys, int* d_source_values, int len, bool isRecor
                                         &ms),
                           ms);
     return ms_or_fail(edpp_sort_i32(c, d_source_values, d_source_values, nullptr, nullptr, len, 0, 32, &ms), ms);
+}
+
+double hashjoin(const int* d_R_keys, const int* d_R_values, int rLen, const int* d_S_keys, const int* d_S_values,
+                int sLen, int& res_len) {
+    (void)d_R_values;
+    (void)d_S_values;
+    edpp_ctx* c = context();
+    if (!c) return -1.0;
+    static std::once_flag once;
+    static uint64_t* d_count = nullptr;  // the process-wide result word, like the context
+    std::call_once(once, [] {
+        if (hipMalloc(&d_count, sizeof(uint64_t)) != hipSuccess) d_count = nullptr;
+    });
+    if (!d_count) return -1.0;
+    double ms = 0.0;
+    const int rc = edpp_hash_join_count(c, d_R_keys, rLen, d_S_keys, sLen, d_count, &ms);
+    uint64_t h = 0;
+    if (rc == EDPP_OK && hipMemcpy(&h, d_count, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) return -1.0;
+    res_len = (int)h;  // the reference's int result
+    return ms_or_fail(rc, ms);
 }
 
 }  // namespace edpp

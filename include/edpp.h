@@ -21,6 +21,7 @@
  *   WG_split          opencl/primitives.h:39-44   -> edpp_split_i32 (EDPP_SPLIT_WG*)
  *   single_split      opencl/primitives.h:46-49   -> edpp_split_i32 (EDPP_SPLIT_SINGLE)
  *   radixSort         obsolete/CUDA/inc/kernels.h:37-45 -> edpp_sort_u32 / edpp_sort_i32
+ *   hashjoin          obsolete/OpenCL/phjImpl.cpp:14-66 -> edpp_hash_join_count
  *   STREAM kernels    opencl/kernels/mem_kernel.cl:9-31 -> edpp_stream_i32
  *   data generators   opencl/util/utility.cpp:214-241 -> edpp_gen_* (reproducible)
  *
@@ -186,6 +187,17 @@ int edpp_sort_u32(edpp_ctx* ctx, const uint32_t* keys_in, uint32_t* keys_out,
 int edpp_sort_i32(edpp_ctx* ctx, const int32_t* keys_in, int32_t* keys_out,
                   const int32_t* vals_in, int32_t* vals_out, int64_t n,
                   int begin_bit, int end_bit, double* ms);
+
+/* ---- partitioned hash join ----------------------------------------------
+ * Replaces the reference's hashjoin (obsolete/OpenCL/phjImpl.cpp:14-66 with
+ * obsolete/OpenCL/hj_partitioned_kernel.cl:7-115), which partitions both
+ * relations with WG_split and counts the joined pairs: *d_count (device
+ * memory, overwritten) = number of pairs (i, j) with r_keys[i] == s_keys[j].
+ * Every int32 key value takes part (the reference reserves -1 as "empty").
+ * The values of the reference's relations do not change the count and are
+ * not taken.                                                                 */
+int edpp_hash_join_count(edpp_ctx* ctx, const int32_t* r_keys, int64_t r_n,
+                         const int32_t* s_keys, int64_t s_n, uint64_t* d_count, double* ms);
 
 /* ---- reproducible synthetic inputs ---------------------------------------
  * Replace opencl/util/utility.cpp:214-241 (time-seeded, schedule-dependent)

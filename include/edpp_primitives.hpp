@@ -72,4 +72,10 @@ double single_split(const int* d_in, int* d_out, int length, int buckets, bool r
 // keys and d_source_keys is unused.
 double radixSort(int* d_source_keys, int* d_source_values, int len, bool isRecord);
 
+// obsolete/OpenCL/phjImpl.cpp:14 (the reference's partitioned hash join):
+// res_len = number of pairs with equal keys (the reference counts only); the
+// value arrays are accepted and do not change the result.  Returns device ms.
+double hashjoin(const int* d_R_keys, const int* d_R_values, int rLen, const int* d_S_keys, const int* d_S_values,
+                int sLen, int& res_len);
+
 }  // namespace edpp

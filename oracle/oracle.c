@@ -248,6 +248,65 @@ void oracle_split(const void* keys_in, void* keys_out, const int32_t* vals_in, i
     free(count);
 }
 
+/* ------------------------------------------------------------------ hash join
+ * Count of equal-key pairs of R and S, the result of the reference's
+ * partitioned hash join (obsolete/OpenCL/phjImpl.cpp:13-66: partition both
+ * relations by key bits, then per partition pair build a hash table of one
+ * side and probe it with the other, counting matches,
+ * obsolete/OpenCL/hj_partitioned_kernel.cl:60-115), restated sequentially:
+ * partition both by the low 8 key bits with a counting sort, then per
+ * partition count matches through a chained hash table of the S side.  Every
+ * key value takes part (the reference's kernel reserves -1 as "empty").      */
+static uint32_t o_mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x85EBCA6Bu;
+    x ^= x >> 13;
+    x *= 0xC2B2AE35u;
+    x ^= x >> 16;
+    return x;
+}
+
+static void o_partition(const uint32_t* k, int64_t n, uint32_t* out, int64_t* start) {
+    int64_t cnt[257] = {0};
+    for (int64_t i = 0; i < n; ++i) cnt[(k[i] & 255u) + 1]++;
+    for (int b = 0; b < 256; ++b) cnt[b + 1] += cnt[b];
+    for (int b = 0; b <= 256; ++b) start[b] = cnt[b];
+    for (int64_t i = 0; i < n; ++i) out[cnt[k[i] & 255u]++] = k[i];
+}
+
+uint64_t oracle_join_count(const int32_t* r, int64_t rn, const int32_t* s, int64_t sn) {
+    uint32_t* rp = (uint32_t*)malloc((size_t)(rn > 0 ? rn : 1) * 4);
+    uint32_t* sp = (uint32_t*)malloc((size_t)(sn > 0 ? sn : 1) * 4);
+    int64_t rst[257], sst[257];
+    o_partition((const uint32_t*)r, rn, rp, rst);
+    o_partition((const uint32_t*)s, sn, sp, sst);
+    uint64_t total = 0;
+    for (int b = 0; b < 256; ++b) {
+        const int64_t s0 = sst[b], s1 = sst[b + 1], r0 = rst[b], r1 = rst[b + 1];
+        if (s1 == s0 || r1 == r0) continue;
+        /* chained table of the S partition: heads by hash, next links */
+        int64_t len = s1 - s0, nb = 1;
+        while (nb < 2 * len) nb <<= 1;
+        int64_t* head = (int64_t*)malloc((size_t)nb * sizeof(int64_t));
+        int64_t* next = (int64_t*)malloc((size_t)len * sizeof(int64_t));
+        for (int64_t h = 0; h < nb; ++h) head[h] = -1;
+        for (int64_t j = 0; j < len; ++j) {
+            const int64_t h = (int64_t)(o_mix32(sp[s0 + j]) & (uint32_t)(nb - 1));
+            next[j] = head[h];
+            head[h] = j;
+        }
+        for (int64_t i = r0; i < r1; ++i) {
+            const int64_t h = (int64_t)(o_mix32(rp[i]) & (uint32_t)(nb - 1));
+            for (int64_t j = head[h]; j >= 0; j = next[j]) total += sp[s0 + j] == rp[i];
+        }
+        free(head);
+        free(next);
+    }
+    free(rp);
+    free(sp);
+    return total;
+}
+
 /* ------------------------------------------------------------------ radix sort
  * LSD radix sort as the reference's radixSort runs it
  * (obsolete/CUDA/kernels/radixSort.cu:20-26: count the digits, exclusive-scan

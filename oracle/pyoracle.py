@@ -44,6 +44,7 @@ def oracle():
             "oracle_scatter": (None, [_P, _P, _P, _I64, ctypes.c_int, ctypes.c_int]),
             "oracle_split": (None, [_P, _P, _P, _P, _P, _I64, ctypes.c_int, ctypes.c_int]),
             "oracle_sort_u32": (None, [_P, _P, _P, _P, _I64, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            "oracle_join_count": (_U64, [_P, _I64, _P, _I64]),
             "oracle_stream": (None, [ctypes.c_int, _P, _P, _P, _I64]),
             "oracle_hampel_mean": (ctypes.c_double, [_P, ctypes.c_int]),
         }
@@ -188,6 +189,13 @@ def sort(keys, vals=None, begin_bit=0, end_bit=32, signed=False):
     oracle().oracle_sort_u32(_a(keys), None if vals is None else _a(np.ascontiguousarray(vals)), _a(ko),
                              None if vo is None else _a(vo), keys.size, begin_bit, end_bit, int(signed))
     return ko, vo
+
+
+def join_count(r, s):
+    """Number of pairs (i, j) with r[i] == s[j] (int32 keys)."""
+    r = np.ascontiguousarray(r, np.int32)
+    s = np.ascontiguousarray(s, np.int32)
+    return int(oracle().oracle_join_count(_a(r), r.size, _a(s), s.size))
 
 
 def stream(op, a, b=None):

@@ -210,6 +210,27 @@ static void test_radix_sort(int len) {
     hipFree(d_vals);
 }
 
+// hashjoin (obsolete/OpenCL/phjImpl.cpp:14): count of equal-key pairs
+static void test_hash_join(int len) {
+    std::vector<int> r(len), s(len / 2 + 3), rv(len, 0), sv(len / 2 + 3, 0);
+    unsigned x = 777u;
+    for (auto& k : r) { x = x * 1664525u + 1013904223u; k = (int)(x >> 12) % 50000; }
+    for (auto& k : s) { x = x * 1664525u + 1013904223u; k = (int)(x >> 12) % 50000; }
+    std::vector<long long> cr(50000, 0), cs(50000, 0);
+    for (int k : r) cr[k]++;
+    for (int k : s) cs[k]++;
+    long long expect = 0;
+    for (int k = 0; k < 50000; ++k) expect += cr[k] * cs[k];
+    int *dr = dev(r), *ds = dev(s), *drv = dev(rv), *dsv = dev(sv);
+    int res = -1;
+    double ms = edpp::hashjoin(dr, drv, len, ds, dsv, (int)s.size(), res);
+    CHECK(ms >= 0 && res == (int)expect, "hashjoin len %d: %d vs %lld", len, res, expect);
+    hipFree(dr);
+    hipFree(ds);
+    hipFree(drv);
+    hipFree(dsv);
+}
+
 int main(int argc, char** argv) {
     int len = argc > 1 ? atoi(argv[1]) : (1 << 20);
     if (!edpp::context()) {
@@ -225,6 +246,7 @@ int main(int argc, char** argv) {
     test_splits(len + 13, 4096);
     test_radix_sort(len);
     test_radix_sort(len / 3 + 1);
+    test_hash_join(len);
     printf("%s: %d failure(s)\n", failures ? "FAILED" : "OK", failures);
     return failures ? 1 : 0;
 }

@@ -372,3 +372,51 @@ def test_sort_full_size_properties(torch_gpu, ctx):
     assert torch.equal(keys[vout.long()], kout)
     assert int(torch.bincount(vout.long(), minlength=n).max()) == 1
     del keys, vals, kout, vout, same
+
+
+# ---------------------------------------------------------------- hash join
+@pytest.mark.parametrize("rn,sn", [(0, 10), (10, 0), (1, 1), (1000, 3000), (100_003, 70_001), (1 << 20, (1 << 20) + 5)])
+@pytest.mark.parametrize("domain", [50, 100_000, 2**31])
+def test_hash_join_count(torch_gpu, ctx, orc, rn, sn, domain):
+    torch = torch_gpu
+    r = (orc.gen_i64(rn, 900 + rn) % (2 * domain) - domain).astype(np.int32)
+    s = (orc.gen_i64(sn, 901 + sn) % (2 * domain) - domain).astype(np.int32)
+    cnt = torch.full((1,), 12345, dtype=torch.int64, device="cuda")
+    ctx.join_count(dev(torch, r) if rn else empty(torch, 0, torch.int32),
+                   dev(torch, s) if sn else empty(torch, 0, torch.int32), cnt, rn, sn)
+    assert int(cnt.item()) == orc.join_count(r, s), (rn, sn, domain)
+
+
+def test_hash_join_skew_and_special_keys(torch_gpu, ctx, orc):
+    torch = torch_gpu
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    # one key everywhere: |R|*|S| > 2^32 pairs, and a partition far above a table fill
+    r = np.full(100_000, -1, np.int32)
+    s = np.full(70_000, -1, np.int32)
+    ctx.join_count(dev(torch, r), dev(torch, s), cnt)
+    assert int(cnt.item()) == 7_000_000_000
+    # structured keys (multiples of 2^20) and the extremes of int32
+    r = (np.arange(200_000, dtype=np.int64) << 20).astype(np.int32)
+    s = np.concatenate([r[::3], np.array([-2**31, 2**31 - 1, -1, 0], np.int32)])
+    ctx.join_count(dev(torch, r), dev(torch, s), cnt)
+    assert int(cnt.item()) == orc.join_count(r, s)
+    # heavy duplicates on both sides
+    r = (orc.gen_i32(300_000, 3) & 63).astype(np.int32)
+    s = (orc.gen_i32(200_000, 4) & 1023).astype(np.int32)
+    ctx.join_count(dev(torch, r), dev(torch, s), cnt)
+    assert int(cnt.item()) == orc.join_count(r, s)
+
+
+def test_hash_join_full_size_property(torch_gpu, ctx):
+    # R = a random permutation of [0, 2^26), S = R's first half shifted by a
+    # multiple of 2^26 plus R's second half: exactly 2^25 matching pairs
+    torch = torch_gpu
+    n = 1 << 26
+    r = empty(torch, n, torch.int32)
+    ctx.gen_perm(r, 77, 0)
+    s = r.clone()
+    s[: n // 2] += n
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    ctx.join_count(r, s, cnt)
+    assert int(cnt.item()) == n // 2
+    del r, s

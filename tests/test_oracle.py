@@ -302,3 +302,34 @@ def test_sort_brute_force_tiny(orc):
         ko, vo = orc.sort(k, v, 0, 3)
         ref = sorted(range(n), key=lambda i: (int(k[i]), i))  # stable by construction
         assert list(vo) == ref and list(ko) == [int(k[i]) for i in ref]
+
+
+# ---------------------------------------------------------------- hash join
+def _join_numpy(r, s):
+    ur, cr = np.unique(r, return_counts=True)
+    us, cs = np.unique(s, return_counts=True)
+    common, ir, is_ = np.intersect1d(ur, us, assume_unique=True, return_indices=True)
+    return int((cr[ir].astype(np.int64) * cs[is_].astype(np.int64)).sum())
+
+
+@pytest.mark.parametrize("domain", [100, 10_000, 2**31])
+def test_join_count_vs_numpy(orc, domain):
+    rng = np.random.default_rng(domain % 97)
+    r = rng.integers(-domain, domain, 30_000).astype(np.int32)
+    s = rng.integers(-domain, domain, 20_000).astype(np.int32)
+    assert orc.join_count(r, s) == _join_numpy(r, s)
+
+
+def test_join_count_brute_force_and_special_keys(orc):
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        r = rng.integers(-3, 3, rng.integers(0, 12)).astype(np.int32)
+        s = rng.integers(-3, 3, rng.integers(0, 12)).astype(np.int32)
+        assert orc.join_count(r, s) == sum(int(a == b) for a in r for b in s)
+    # -1 and INT_MIN are ordinary keys; one key everywhere gives |R| * |S|
+    r = np.full(3000, -1, np.int32)
+    s = np.full(2000, -1, np.int32)
+    assert orc.join_count(r, s) == 6_000_000
+    r = np.array([-2**31, -1, 0, 2**31 - 1], np.int32)
+    assert orc.join_count(r, r) == 4
+    assert orc.join_count(r, np.empty(0, np.int32)) == 0
