@@ -41,6 +41,8 @@ def parse():
     p.add_argument("--suite", action="store_true", help="measure every primitive too")
     p.add_argument("--batched", action="store_true", help="10,000-job mix: one persistent launch vs per-call")
     p.add_argument("--learned", action="store_true", help="train the schedule model on autotuner timings")
+    p.add_argument("--access", action="store_true",
+                   help="access-pattern (row/column/mixed x repeat) and workgroup-order (chunk x order x cache) sweeps")
     p.add_argument("--train-calls", type=int, default=2000)
     p.add_argument("--test-calls", type=int, default=500)
     p.add_argument("--db-out", default=None, help="write the autotuner database (JSON lines) here")
@@ -323,9 +325,11 @@ def main():
     del dst2
     if args.suite:
         line["suite"] = suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs)
-    if args.batched or args.learned:
+    if args.batched or args.learned or args.access:
         del src, dst, idx
         torch.cuda.empty_cache()
+    if args.access:
+        line["access"] = access(ctx, edpp, torch, T)
     if args.batched:
         line["batched"] = batched(ctx, edpp, torch)
     if args.learned:
@@ -420,6 +424,58 @@ def suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs):
     print("hash_join", jn, file=sys.stderr, flush=True)
     del ko, vo, b
     ctx.gen_i32(src, 0xDA7A)
+    return res
+
+
+def access(ctx, edpp, torch, T):
+    """The reference's access studies (opencl/test/test_access.cpp,
+    opencl/test/test_access_wg.cpp) on this GPU: out = 3*in through the
+    column / row / mixed lane mappings for repeat 1..20 over 512*32768 lanes,
+    and chunked reads of 2^28 int32 per workgroup chunk size, in interleaved or
+    random workgroup order, with the data cold (a 512 MiB write flushed the
+    caches) or loaded (just written)."""
+    i32 = torch.int32
+    lanes = 512 * 32768
+    big = torch.empty(lanes * 20, dtype=i32, device="cuda")
+    ctx.gen_i32(big, 0xACC)
+    out = torch.empty_like(big)
+    res = {"scale": {}}
+    for pname, pat in (("column", edpp.ACCESS_COLUMN), ("row", edpp.ACCESS_ROW), ("mixed", edpp.ACCESS_MIXED)):
+        row = {}
+        for rep in (1, 2, 4, 8, 16, 20):
+            n = lanes * rep
+            t = T.time(lambda: ctx.access_scale(pat, big, out, rep, n=n), reps=5)
+            row[rep] = round(gbs(8.0 * n, t["hampel_ms"]), 1)
+        res["scale"][pname] = row
+        print("access scale", pname, row, file=sys.stderr, flush=True)
+    del big, out
+    n = N_FULL
+    data = torch.empty(n, dtype=i32, device="cuda")
+    ctx.gen_i32(data, 0xACD)
+    flush = torch.empty(1 << 27, dtype=i32, device="cuda")  # 512 MiB: twice the Infinity Cache
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    expect = None
+    res["chunks"] = {}
+    for kb in (8, 64, 512, 4096, 8192):
+        chunk = kb * 256  # int32 elements in kb KiB
+        chunks = -(-n // chunk)
+        orders = {"interleaved": torch.arange(chunks, dtype=i32, device="cuda"),
+                  "random": torch.randperm(chunks, device="cuda").to(i32)}
+        for oname, order in orders.items():
+            for cname in ("cold", "loaded"):
+                ms = []
+                for _ in range(5):
+                    if cname == "cold":
+                        flush.fill_(1)
+                    else:
+                        ctx.stream_op(edpp.STREAM_COPY, data, None, data, n)  # rewrite: the tail is cached
+                    ms.append(ctx.access_chunks(data, chunk, order, cnt, timed=True))
+                c = int(cnt.item())
+                expect = c if expect is None else expect
+                assert c == expect, "chunk count changed with the order"
+                res["chunks"][f"{kb}KiB_{oname}_{cname}"] = round(gbs(4.0 * n, hampel(ms)), 1)
+        print("access chunks", kb, {k: v for k, v in res["chunks"].items() if k.startswith(f"{kb}KiB")},
+              file=sys.stderr, flush=True)
     return res
 
 

@@ -22,6 +22,7 @@ SPLIT_AUTO, SPLIT_WG, SPLIT_WI, SPLIT_SINGLE = 0, 1, 2, 3
 STREAM_COPY, STREAM_SCALE, STREAM_ADD, STREAM_TRIAD = 0, 1, 2, 3
 PERM_RANDOM, PERM_BLOCK_LOCAL, PERM_IDENTITY = 0, 1, 2
 KEYS_UNIFORM, KEYS_ZIPF = 0, 1
+ACCESS_COLUMN, ACCESS_ROW, ACCESS_MIXED = 0, 1, 2
 
 _lib = None
 
@@ -46,6 +47,8 @@ _SIGS = {
     "edpp_scatter_i64": [c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_dp],
     "edpp_default_passes": [c_vp, c_i64, c_int, c_int, ctypes.POINTER(c_int)],
     "edpp_hash_join_count": [c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_dp],
+    "edpp_access_scale_i32": [c_vp, c_int, c_vp, c_vp, c_i64, c_int, c_dp],
+    "edpp_access_chunks_i32": [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_dp],
     "edpp_sort_u32": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_dp],
     "edpp_sort_i32": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_dp],
     "edpp_scan_exclusive_i32": [c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_dp],
@@ -269,6 +272,16 @@ class Context:
         name = "edpp_sort_i32" if signed else "edpp_sort_u32"
         return self._call(name, _ptr(keys_in), _ptr(keys_out), _ptr(vals_in), _ptr(vals_out), n, int(begin_bit),
                           int(end_bit), timed=timed)
+
+    def access_scale(self, pattern, src, dst, repeat, n=None, timed=False):
+        """dst = 3 * src through the pattern's lane mapping (n a multiple of 64*repeat)."""
+        n = _numel(src) if n is None else n
+        return self._call("edpp_access_scale_i32", int(pattern), _ptr(src), _ptr(dst), n, int(repeat), timed=timed)
+
+    def access_chunks(self, src, chunk, order, count, n=None, timed=False):
+        """count[0] = odd elements of src, read in chunks taken in `order`."""
+        n = _numel(src) if n is None else n
+        return self._call("edpp_access_chunks_i32", _ptr(src), n, int(chunk), _ptr(order), _ptr(count), timed=timed)
 
     def join_count(self, r_keys, s_keys, count, r_n=None, s_n=None, timed=False):
         """count[0] (uint64/int64 device tensor) = number of equal-key pairs of R and S."""

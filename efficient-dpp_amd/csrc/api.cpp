@@ -485,6 +485,32 @@ int edpp_sort_i32(edpp_ctx* c, const int32_t* keys_in, int32_t* keys_out, const 
                        ms);
 }
 
+// ---------------------------------------------------------------- access probes
+int edpp_access_scale_i32(edpp_ctx* c, int pattern, const int32_t* in, int32_t* out, int64_t n, int repeat,
+                          double* ms) {
+    EDPP_CHECK_CTX(c);
+    if (pattern < EDPP_ACCESS_COLUMN || pattern > EDPP_ACCESS_MIXED)
+        return fail(c, EDPP_EINVAL, "unknown access pattern %d", pattern);
+    if (repeat < 1 || n < 0 || n % ((int64_t)repeat * 64) != 0)
+        return fail(c, EDPP_EINVAL, "n=%lld must be a multiple of 64*repeat (repeat=%d)", (long long)n, repeat);
+    if (n > 0 && (!in || !out)) return fail(c, EDPP_EINVAL, "null buffer");
+    int rc = timing_begin(c, ms);
+    if (!rc) rc = launch_access_scale(c, pattern, in, out, n, repeat);
+    if (!rc) rc = timing_end(c, ms);
+    return rc;
+}
+int edpp_access_chunks_i32(edpp_ctx* c, const int32_t* in, int64_t n, int64_t chunk, const int32_t* order,
+                           int64_t* d_count, double* ms) {
+    EDPP_CHECK_CTX(c);
+    if (n < 0 || chunk < 1 || !d_count || (n > 0 && (!in || !order)))
+        return fail(c, EDPP_EINVAL, "access chunks: bad argument");
+    if (ceil_div(n, chunk) >= (1ll << 31)) return fail(c, EDPP_EINVAL, "too many chunks");
+    int rc = timing_begin(c, ms);
+    if (!rc) rc = launch_access_chunks(c, in, n, chunk, order, (unsigned long long*)d_count);
+    if (!rc) rc = timing_end(c, ms);
+    return rc;
+}
+
 // ---------------------------------------------------------------- hash join
 int edpp_hash_join_count(edpp_ctx* c, const int32_t* r_keys, int64_t r_n, const int32_t* s_keys, int64_t s_n,
                          uint64_t* d_count, double* ms) {

@@ -23,6 +23,7 @@
  *   radixSort         obsolete/CUDA/inc/kernels.h:37-45 -> edpp_sort_u32 / edpp_sort_i32
  *   hashjoin          obsolete/OpenCL/phjImpl.cpp:14-66 -> edpp_hash_join_count
  *   STREAM kernels    opencl/kernels/mem_kernel.cl:9-31 -> edpp_stream_i32
+ *   access probes     opencl/kernels/mem_kernel.cl:33-86 -> edpp_access_scale_i32 / edpp_access_chunks_i32
  *   data generators   opencl/util/utility.cpp:214-241 -> edpp_gen_* (reproducible)
  *
  * All results are bit-exact with the sequential definitions:
@@ -187,6 +188,29 @@ int edpp_sort_u32(edpp_ctx* ctx, const uint32_t* keys_in, uint32_t* keys_out,
 int edpp_sort_i32(edpp_ctx* ctx, const int32_t* keys_in, int32_t* keys_out,
                   const int32_t* vals_in, int32_t* vals_out, int64_t n,
                   int begin_bit, int end_bit, double* ms);
+
+/* ---- access-pattern and workgroup-order probes ------------------------------
+ * opencl/kernels/mem_kernel.cl:33-86 with opencl/test/test_access.cpp and
+ * opencl/test/test_access_wg.cpp.
+ * edpp_access_scale_i32: out[i] = 3 * in[i] for i < n, touched by
+ * lanes = n / repeat lanes in the pattern's mapping (lane t, step k):
+ *   EDPP_ACCESS_COLUMN  t + k*lanes               (scale_column, coalesced)
+ *   EDPP_ACCESS_ROW     t*repeat + k              (scale_row, a run per lane)
+ *   EDPP_ACCESS_MIXED   (t/64)*64*repeat + k*64 + t%64   (scale_mixed, a run per wave)
+ * n must be a multiple of 64*repeat, repeat >= 1.
+ * edpp_access_chunks_i32: workgroups take chunks of `chunk` elements in the
+ * order order[0], order[1], ... (a permutation of the ceil(n/chunk) chunk ids,
+ * device memory) and *d_count (device) = number of odd elements of in
+ * (wg_access; its output is racy in the reference, here it is exact).      */
+enum edpp_access_pattern {
+    EDPP_ACCESS_COLUMN = 0,
+    EDPP_ACCESS_ROW    = 1,
+    EDPP_ACCESS_MIXED  = 2
+};
+int edpp_access_scale_i32(edpp_ctx* ctx, int pattern, const int32_t* in, int32_t* out, int64_t n,
+                          int repeat, double* ms);
+int edpp_access_chunks_i32(edpp_ctx* ctx, const int32_t* in, int64_t n, int64_t chunk,
+                           const int32_t* order, int64_t* d_count, double* ms);
 
 /* ---- partitioned hash join ----------------------------------------------
  * Replaces the reference's hashjoin (obsolete/OpenCL/phjImpl.cpp:14-66 with

@@ -420,3 +420,36 @@ def test_hash_join_full_size_property(torch_gpu, ctx):
     ctx.join_count(r, s, cnt)
     assert int(cnt.item()) == n // 2
     del r, s
+
+
+# ---------------------------------------------------------------- access probes
+@pytest.mark.parametrize("pattern", [0, 1, 2])
+@pytest.mark.parametrize("repeat", [1, 2, 3, 7, 20])
+@pytest.mark.parametrize("k", [1, 37, 4096])
+def test_access_scale(torch_gpu, ctx, orc, pattern, repeat, k):
+    torch = torch_gpu
+    n = 64 * repeat * k
+    a = orc.gen_i32(n, 31 + n)
+    out = torch.zeros(n, dtype=torch.int32, device="cuda")
+    ctx.access_scale(pattern, dev(torch, a), out, repeat)
+    assert np.array_equal(host(out), orc.stream(1, a)), (pattern, repeat, k)
+
+
+def test_access_scale_rejects_ragged(torch_gpu, ctx, edpp):
+    torch = torch_gpu
+    a = torch.zeros(64 * 3 + 1, dtype=torch.int32, device="cuda")
+    with pytest.raises(edpp.EdppError):
+        ctx.access_scale(0, a, a.clone(), 3)
+
+
+@pytest.mark.parametrize("n", [1, 1000, (1 << 20) + 3])
+@pytest.mark.parametrize("chunk", [1, 256, 4096, 1 << 21])
+def test_access_chunks(torch_gpu, ctx, orc, n, chunk):
+    torch = torch_gpu
+    a = orc.gen_i32(n, 71 + n)
+    chunks = -(-n // chunk)
+    cnt = torch.full((1,), -5, dtype=torch.int64, device="cuda")
+    for order in (np.arange(chunks), np.arange(chunks)[::-1],
+                  np.random.default_rng(chunks).permutation(chunks)):
+        ctx.access_chunks(dev(torch, a), chunk, dev(torch, np.ascontiguousarray(order, np.int32)), cnt)
+        assert int(cnt.item()) == int((a & 1).sum()), (n, chunk)
