@@ -52,7 +52,8 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=1 << 26, help="elements in the CPU baseline sample")
     p.add_argument("--passes", type=int, default=0, help="pass count of the default step (0 = library schedule)")
     p.add_argument("--out", default=None, help="also write the JSON line to this file")
-    p.add_argument("--profile", default=None, choices=["stream", "gather", "scatter", "scan", "split", "batched"],
+    p.add_argument("--profile", default=None,
+                   choices=["stream", "gather", "scatter", "scan", "split", "sort", "join", "batched"],
                    help="only run one workload at full size (for rocprofv3), plus one STREAM copy")
     return p.parse_args()
 
@@ -203,6 +204,18 @@ def profile_only(args, ctx, edpp, torch):
         ctx.gen_split_keys(a, v, 256, edpp.KEYS_UNIFORM, 0x5917)
         for _ in range(reps):
             ctx.split(a, b, 256, v, vo, None, n, edpp.SPLIT_AUTO)
+    elif args.profile == "sort":
+        v = torch.arange(n, dtype=i32, device="cuda")
+        vo = torch.empty(n, dtype=i32, device="cuda")
+        for _ in range(reps):
+            ctx.sort(a, b, v, vo, n, 0, 32)
+    elif args.profile == "join":
+        m = 1 << 24
+        ctx.gen_perm(a[:m], 0x1A, edpp.PERM_RANDOM, n=m)
+        ctx.gen_perm(b[:m], 0x1B, edpp.PERM_RANDOM, n=m)
+        cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+        for _ in range(reps):
+            ctx.join_count(a[:m], b[:m], cnt, m, m)
     elif args.profile == "batched":
         del a, b
         batched(ctx, edpp, torch)

@@ -203,19 +203,22 @@ __global__ void k_split_single_scatter(const K* kin, K* kout, const int32_t* vin
     }
 }
 
-// counts[b] += number of keys with ((key >> 0) & (B-1)) == b (order-free)
+// d_start[b] = first position of bucket b in the split output kout, which is
+// ordered by bucket: a binary search per bucket (a bucket absent from the
+// output starts where the next present one does, or at n)
 template <class K>
-__global__ __launch_bounds__(256) void k_bucket_count(const K* __restrict__ kin, int64_t n, int lg,
-                                                      uint32_t* __restrict__ counts) {
-    extern __shared__ uint32_t h[];
+__global__ void k_split_start_search(const K* __restrict__ kout, int64_t n, int lg, int32_t* __restrict__ d_start) {
     const int B = 1 << lg;
-    for (int b = threadIdx.x; b < B; b += blockDim.x) h[b] = 0;
-    __syncthreads();
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        atomicAdd(&h[(uint32_t)kin[i] & (uint32_t)(B - 1)], 1u);
-    __syncthreads();
-    for (int b = threadIdx.x; b < B; b += blockDim.x)
-        if (h[b]) atomicAdd(&counts[b], h[b]);
+    const uint32_t mask = (uint32_t)B - 1u;
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < B; b += gridDim.x * blockDim.x) {
+        int64_t lo = 0, hi = n;
+        while (lo < hi) {
+            const int64_t mid = lo + (hi - lo) / 2;
+            if ((((uint32_t)kout[mid]) & mask) < (uint32_t)b) lo = mid + 1;
+            else hi = mid;
+        }
+        d_start[b] = (int32_t)lo;
+    }
 }
 
 __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
@@ -456,9 +459,8 @@ int launch_split(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t
     const size_t hbytes = ((size_t)hlen * 4 + 255) & ~(size_t)255;
     const size_t kbytes = two ? (((size_t)n * sizeof(K) + 255) & ~(size_t)255) : 0;
     const size_t vbytes = (two && vin) ? (((size_t)n * 4 + 255) & ~(size_t)255) : 0;
-    const size_t cbytes = (two && d_start) ? 256 * 64 : 0;  // 4096 counters
     void* w;
-    int rc = ws_reserve(c, hbytes + kbytes + vbytes + cbytes, &w);
+    int rc = ws_reserve(c, hbytes + kbytes + vbytes, &w);
     if (rc) return rc;
     char* base = static_cast<char*>(w);
     int32_t* his = reinterpret_cast<int32_t*>(base);
@@ -475,14 +477,9 @@ int launch_split(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t
     rc = run_pass<K>(c, strategy, a2, his);
     if (rc) return rc;
     if (d_start) {
-        uint32_t* cnt = reinterpret_cast<uint32_t*>(base + hbytes + kbytes + vbytes);
-        hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)ceil_div(buckets, 256)), dim3(256), 0, c->stream, cnt,
-                           (int64_t)buckets, 0u);
-        const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(n, 256), (int64_t)c->cus * 4);
-        hipLaunchKernelGGL((k_bucket_count<K>), dim3(grid), dim3(256), (size_t)buckets * 4, c->stream, kin, n, lg, cnt);
+        hipLaunchKernelGGL((k_split_start_search<K>), dim3((unsigned)ceil_div(buckets, 256)), dim3(256), 0, c->stream,
+                           kout, n, lg, d_start);
         EDPP_HIP(c, hipGetLastError());
-        rc = launch_scan<uint32_t>(c, cnt, (uint32_t*)d_start, buckets, EDPP_SCAN_LOOKBACK, nullptr);
-        if (rc) return rc;
     }
     return EDPP_OK;
 }
