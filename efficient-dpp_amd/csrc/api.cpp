@@ -53,6 +53,22 @@ int ws_outer_reserve(edpp_ctx* c, size_t bytes, void** out) {
     return grow(c, &c->ws_outer, &c->ws_outer_bytes, bytes, out);
 }
 
+// Zeroes `bytes` (a multiple of 4) of device memory with a kernel on the
+// context's stream: a kernel node, unlike a memset node, is safe to capture
+// into a graph and replay.
+__global__ void k_zero_words(uint32_t* p, int64_t words) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+int zero_fill(edpp_ctx* c, void* p, size_t bytes) {
+    const int64_t words = (int64_t)(bytes / 4);
+    if (words == 0) return EDPP_OK;
+    const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(words, 256), (int64_t)std::max(c->cus, 1) * 8);
+    hipLaunchKernelGGL(k_zero_words, dim3(grid), dim3(256), 0, c->stream, static_cast<uint32_t*>(p), words);
+    EDPP_HIP(c, hipGetLastError());
+    return EDPP_OK;
+}
+
 int scan_state_acquire(edpp_ctx* c, int64_t tiles, uint32_t* epoch_out) {
     if (tiles > c->scan_cap_tiles) {
         int64_t cap = std::max<int64_t>(tiles, 1 << 16);
@@ -68,17 +84,19 @@ int scan_state_acquire(edpp_ctx* c, int64_t tiles, uint32_t* epoch_out) {
             c->scan_cap_tiles = 0;
             return fail(c, EDPP_ENOMEM, "look-back state for %lld tiles", (long long)cap);
         }
-        EDPP_HIP(c, hipMemsetAsync(c->scan_status, 0, cap * sizeof(uint64_t), c->stream));
-        EDPP_HIP(c, hipMemsetAsync(c->scan_status64, 0, cap * 2 * sizeof(uint64_t), c->stream));
-        EDPP_HIP(c, hipMemsetAsync(c->tickets, 0, 2 * sizeof(uint32_t), c->stream));
+        int rc = zero_fill(c, c->scan_status, cap * sizeof(uint64_t));
+        if (!rc) rc = zero_fill(c, c->scan_status64, cap * 2 * sizeof(uint64_t));
+        if (!rc) rc = zero_fill(c, c->tickets, 2 * sizeof(uint32_t));
+        if (rc) return rc;
         c->scan_cap_tiles = cap;
         c->epoch = 1;
     }
     if (c->epoch >= (1u << 30) - 1) {
         // epochs are 30-bit: clear the state once per 2^30 calls
-        EDPP_HIP(c, hipMemsetAsync(c->scan_status, 0, c->scan_cap_tiles * sizeof(uint64_t), c->stream));
-        EDPP_HIP(c, hipMemsetAsync(c->scan_status64, 0, c->scan_cap_tiles * 2 * sizeof(uint64_t), c->stream));
-        EDPP_HIP(c, hipMemsetAsync(c->tickets, 0, 2 * sizeof(uint32_t), c->stream));
+        int rc = zero_fill(c, c->scan_status, c->scan_cap_tiles * sizeof(uint64_t));
+        if (!rc) rc = zero_fill(c, c->scan_status64, c->scan_cap_tiles * 2 * sizeof(uint64_t));
+        if (!rc) rc = zero_fill(c, c->tickets, 2 * sizeof(uint32_t));
+        if (rc) return rc;
         c->epoch = 1;
     }
     *epoch_out = c->epoch++;

@@ -62,6 +62,7 @@ int hip_fail(edpp_ctx* c, hipError_t e, const char* what);
 // workspace of at least `bytes` (256-byte aligned); may synchronise to grow
 int ws_reserve(edpp_ctx* c, size_t bytes, void** out);
 int ws_outer_reserve(edpp_ctx* c, size_t bytes, void** out);
+int zero_fill(edpp_ctx* c, void* p, size_t bytes);  // kernel, graph-capturable
 // look-back state for at least `tiles` tiles; returns the epoch for this call
 int scan_state_acquire(edpp_ctx* c, int64_t tiles, uint32_t* epoch_out);
 

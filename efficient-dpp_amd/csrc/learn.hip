@@ -542,8 +542,9 @@ int edpp_mlp_set_params(edpp_mlp* m, const float* host, int64_t count) {
     if (!m || !host || count != NPARAM) return EDPP_EINVAL;
     edpp_ctx* c = m->ctx;
     EDPP_HIP(c, hipMemcpyAsync(m->params, host, sizeof(float) * NPARAM, hipMemcpyHostToDevice, c->stream));
-    EDPP_HIP(c, hipMemsetAsync(m->adam_m, 0, sizeof(float) * NPARAM, c->stream));
-    EDPP_HIP(c, hipMemsetAsync(m->adam_v, 0, sizeof(float) * NPARAM, c->stream));
+    int zrc = zero_fill(c, m->adam_m, sizeof(float) * NPARAM);
+    if (!zrc) zrc = zero_fill(c, m->adam_v, sizeof(float) * NPARAM);
+    if (zrc) return zrc;
     EDPP_HIP(c, hipStreamSynchronize(c->stream));
     m->adam_step = 0;
     return EDPP_OK;

@@ -280,7 +280,7 @@ int run_rts_ssa(edpp_ctx* c, const T* in, T* out, int64_t n, T* d_total, bool ss
 template <class T>
 int launch_scan(edpp_ctx* c, const T* in, T* out, int64_t n, int scheme, T* d_total) {
     if (n == 0) {
-        if (d_total) EDPP_HIP(c, hipMemsetAsync(d_total, 0, sizeof(T), c->stream));
+        if (d_total) return zero_fill(c, d_total, sizeof(T));
         return EDPP_OK;
     }
     if (!aligned16(in) || !aligned16(out)) return run_unaligned<T>(c, in, out, n, d_total);

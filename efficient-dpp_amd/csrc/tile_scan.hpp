@@ -29,7 +29,6 @@ struct TileScan {
     };
 
     T v[VPT][EPV];
-    T s[VPT];     // per-lane sum of sub-tile j
     T off[VPT];   // exclusive offset of the lane's first element of sub-tile j, within the tile
     T total;      // tile sum
 
@@ -80,19 +79,20 @@ struct TileScan {
         }
     }
 
-    // Block-wide scan of the tile; contains one __syncthreads.
+    // Block-wide scan of the tile; contains one __syncthreads.  off[j] first
+    // holds the lane's exclusive prefix inside its wave, then gets the
+    // earlier waves and sub-tiles added (one array of VPT values per lane).
     __device__ __forceinline__ void scan(Smem& sm) {
         const int lane = lane_id();
         const int w = threadIdx.x / kWave;
-        T inc[VPT];
         #pragma unroll
         for (int j = 0; j < VPT; ++j) {
             T a = 0;
             #pragma unroll
             for (int k = 0; k < EPV; ++k) a += v[j][k];
-            s[j] = a;
-            inc[j] = wave_inclusive_add(a);
-            if (lane == kWave - 1) sm.wave_tot[j][w] = inc[j];
+            const T inc = wave_inclusive_add(a);
+            off[j] = inc - a;
+            if (lane == kWave - 1) sm.wave_tot[j][w] = inc;
         }
         __syncthreads();
         T run = 0;
@@ -105,7 +105,7 @@ struct TileScan {
                 before += (q < w) ? x : T(0);
                 sub += x;
             }
-            off[j] = run + before + inc[j] - s[j];
+            off[j] += run + before;
             run += sub;
         }
         total = run;
