@@ -10,6 +10,7 @@
 // MI355X layout: each lane owns U groups of 4 consecutive positions; the
 // index (and, for scatter, value) groups are read as 16-byte vectors with
 // non-temporal hints so the streaming side does not evict the random side.
+#include <algorithm>
 #include "ctx.hpp"
 #include "device.hpp"
 
@@ -44,28 +45,102 @@ __device__ __forceinline__ void store4(T* p, const T (&v)[4]) {
     }
 }
 
+// Window staging.  A workgroup's chunk is 1024*U consecutive positions.  When
+// the in-slice indices of the chunk fall inside a window of kGsWin elements
+// (a block-local permutation: shuffled 4096-element blocks, shuffled inside),
+// the random side goes through LDS: a gather loads the window with
+// consecutive loads and reads its values from LDS; a scatter writes its values
+// into the window and stores the covered positions as consecutive runs, so
+// the lines of the block leave whole instead of as 4-byte pieces.  Otherwise
+// the random side is addressed directly.
+//
+// With several passes the window test runs in the first pass over all of
+// the chunk's indices in [0, n); a chunk that fits is done there at once
+// (each position is still written exactly once) and marked in done[], and the
+// later passes skip it without reading its indices.  A block-local
+// permutation so costs one pass whatever the pass count.
+constexpr int kGsWin = 4096;
+enum GsPhase { kGsOnly = 0, kGsFirst = 1, kGsLater = 2 };
+
+// Smallest and largest index in [0, nall) of the workgroup's chunk (lo > hi
+// when there is none), the same in every lane.  Ends with a barrier.
+template <int U>
+__device__ __forceinline__ void chunk_window(const uint32_t (&pos)[U][4], uint32_t nall, uint32_t* wlo, uint32_t* whi,
+                                             uint32_t& lo, uint32_t& hi) {
+    lo = 0xFFFFFFFFu;
+    hi = 0;
+    #pragma unroll
+    for (int u = 0; u < U; ++u)
+        #pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (pos[u][k] < nall) {
+                lo = std::min(lo, pos[u][k]);
+                hi = std::max(hi, pos[u][k]);
+            }
+    #pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        lo = std::min(lo, (uint32_t)__shfl_xor((int)lo, o));
+        hi = std::max(hi, (uint32_t)__shfl_xor((int)hi, o));
+    }
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0) {
+        wlo[w] = lo;
+        whi[w] = hi;
+    }
+    __syncthreads();
+    #pragma unroll
+    for (int q = 0; q < kGsBlock / kWave; ++q) {
+        lo = std::min(lo, wlo[q]);
+        hi = std::max(hi, whi[q]);
+    }
+}
+
 // out[i] = in[idx[i]] for idx[i] in [from, from+span).  The host clamps
 // from+span to n, so an index outside [0,n) is skipped instead of read.
 template <class T, int U, bool RNT>
 __global__ __launch_bounds__(kGsBlock) void k_gather(const T* __restrict__ in, T* __restrict__ out,
                                                      const int32_t* __restrict__ idx, int64_t ngrp, uint32_t from,
-                                                     uint32_t span) {
+                                                     uint32_t span, uint32_t nall, uint8_t* __restrict__ done,
+                                                     int phase) {
+    __shared__ T win[kGsWin];
+    __shared__ uint32_t wlo[kGsBlock / kWave], whi[kGsBlock / kWave];
+    if (phase == kGsLater && done[blockIdx.x]) return;  // the same in the whole workgroup
     const int64_t g0 = (int64_t)blockIdx.x * kGsBlock * U + threadIdx.x;
     uint32_t pos[U][4];
     #pragma unroll
     for (int u = 0; u < U; ++u) {
         int64_t g = g0 + (int64_t)u * kGsBlock;
-        v4u32 x = {from + span, from + span, from + span, from + span};  // out of range
+        v4u32 x = {nall, nall, nall, nall};  // out of range
         if (g < ngrp) x = ld_stream(reinterpret_cast<const v4u32*>(idx) + g);
         #pragma unroll
         for (int k = 0; k < 4; ++k) pos[u][k] = x[k];
     }
     T val[U][4];
-    #pragma unroll
-    for (int u = 0; u < U; ++u)
+    bool window = false;
+    if (phase != kGsLater) {
+        uint32_t lo, hi;
+        chunk_window<U>(pos, nall, wlo, whi, lo, hi);
+        window = lo <= hi && hi - lo < (uint32_t)kGsWin;  // the same in the whole workgroup
+        if (phase == kGsFirst && threadIdx.x == 0) done[blockIdx.x] = window ? 1 : 0;
+        if (window) {
+            for (uint32_t o = threadIdx.x; o <= hi - lo; o += kGsBlock) win[o] = in[lo + o];
+            __syncthreads();
+            #pragma unroll
+            for (int u = 0; u < U; ++u)
+                #pragma unroll
+                for (int k = 0; k < 4; ++k) val[u][k] = pos[u][k] < nall ? win[pos[u][k] - lo] : T(0);
+            // every index of the chunk is done: the slice of this pass is all of [0, n)
+            from = 0;
+            span = nall;
+        }
+    }
+    if (!window) {
         #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            val[u][k] = (pos[u][k] - from < span) ? (RNT ? ld_stream(in + pos[u][k]) : in[pos[u][k]]) : T(0);
+        for (int u = 0; u < U; ++u)
+            #pragma unroll
+            for (int k = 0; k < 4; ++k)
+                val[u][k] = (pos[u][k] - from < span) ? (RNT ? ld_stream(in + pos[u][k]) : in[pos[u][k]]) : T(0);
+    }
     #pragma unroll
     for (int u = 0; u < U; ++u) {
         int64_t g = g0 + (int64_t)u * kGsBlock;
@@ -88,7 +163,13 @@ __global__ __launch_bounds__(kGsBlock) void k_gather(const T* __restrict__ in, T
 template <class T, int U, bool RNT>
 __global__ __launch_bounds__(kGsBlock) void k_scatter(const T* __restrict__ in, T* __restrict__ out,
                                                       const int32_t* __restrict__ idx, int64_t ngrp, uint32_t from,
-                                                      uint32_t span) {
+                                                      uint32_t span, uint32_t nall, uint8_t* __restrict__ done,
+                                                      int phase) {
+    __shared__ T win[kGsWin];
+    __shared__ uint32_t cov[kGsWin / 32];  // positions of the window this chunk writes
+    __shared__ uint32_t wlo[kGsBlock / kWave], whi[kGsBlock / kWave];
+    if (phase == kGsLater && done[blockIdx.x]) return;  // the same in the whole workgroup
+    for (int i = threadIdx.x; i < kGsWin / 32; i += kGsBlock) cov[i] = 0;
     const int64_t g0 = (int64_t)blockIdx.x * kGsBlock * U + threadIdx.x;
     uint32_t pos[U][4];
     T val[U][4];
@@ -102,17 +183,40 @@ __global__ __launch_bounds__(kGsBlock) void k_scatter(const T* __restrict__ in, 
             load4(in + 4 * g, val[u]);
         } else {
             #pragma unroll
-            for (int k = 0; k < 4; ++k) { pos[u][k] = from + span; val[u][k] = 0; }
+            for (int k = 0; k < 4; ++k) { pos[u][k] = nall; val[u][k] = 0; }
         }
     }
-    #pragma unroll
-    for (int u = 0; u < U; ++u)
+    bool window = false;
+    if (phase != kGsLater) {
+        uint32_t lo, hi;
+        chunk_window<U>(pos, nall, wlo, whi, lo, hi);  // its barrier also orders the cov clearing
+        window = lo <= hi && hi - lo < (uint32_t)kGsWin;  // the same in the whole workgroup
+        if (phase == kGsFirst && threadIdx.x == 0) done[blockIdx.x] = window ? 1 : 0;
+        if (window) {
+            #pragma unroll
+            for (int u = 0; u < U; ++u)
+                #pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (pos[u][k] < nall) {
+                        const uint32_t o = pos[u][k] - lo;
+                        win[o] = val[u][k];
+                        atomicOr(cov + (o >> 5), 1u << (o & 31));
+                    }
+            __syncthreads();
+            for (uint32_t o = threadIdx.x; o <= hi - lo; o += kGsBlock)
+                if ((cov[o >> 5] >> (o & 31)) & 1u) out[lo + o] = win[o];
+        }
+    }
+    if (!window) {
         #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (pos[u][k] - from < span) {
-                if (RNT) st_stream(val[u][k], out + pos[u][k]);
-                else out[pos[u][k]] = val[u][k];
-            }
+        for (int u = 0; u < U; ++u)
+            #pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (pos[u][k] - from < span) {
+                    if (RNT) st_stream(val[u][k], out + pos[u][k]);
+                    else out[pos[u][k]] = val[u][k];
+                }
+    }
 }
 
 // elements [lo, n): scalar tail and the path for unaligned pointers
@@ -129,26 +233,32 @@ __global__ void k_gs_scalar(const T* in, T* out, const int32_t* idx, int64_t lo,
     }
 }
 
+struct GsPass {
+    uint32_t from, span, nall;
+    uint8_t* done;  // one flag per workgroup chunk, written by the first of several passes
+    int phase;
+};
+
 template <class T, int U, bool RNT>
-static void launch_gs_kernel(edpp_ctx* c, const T* in, T* out, const int32_t* idx, int64_t ngrp, uint32_t from,
-                             uint32_t span, bool scatter) {
+static void launch_gs_kernel(edpp_ctx* c, const T* in, T* out, const int32_t* idx, int64_t ngrp, const GsPass& q,
+                             bool scatter) {
     const unsigned grid = (unsigned)ceil_div(ngrp, (int64_t)kGsBlock * U);
     if (scatter)
-        hipLaunchKernelGGL((k_scatter<T, U, RNT>), dim3(grid), dim3(kGsBlock), 0, c->stream, in, out, idx, ngrp, from,
-                           span);
+        hipLaunchKernelGGL((k_scatter<T, U, RNT>), dim3(grid), dim3(kGsBlock), 0, c->stream, in, out, idx, ngrp,
+                           q.from, q.span, q.nall, q.done, q.phase);
     else
-        hipLaunchKernelGGL((k_gather<T, U, RNT>), dim3(grid), dim3(kGsBlock), 0, c->stream, in, out, idx, ngrp, from,
-                           span);
+        hipLaunchKernelGGL((k_gather<T, U, RNT>), dim3(grid), dim3(kGsBlock), 0, c->stream, in, out, idx, ngrp,
+                           q.from, q.span, q.nall, q.done, q.phase);
 }
 
 template <class T, bool RNT>
-static void dispatch_unroll(edpp_ctx* c, int u, const T* in, T* out, const int32_t* idx, int64_t ngrp, uint32_t from,
-                            uint32_t span, bool scatter) {
+static void dispatch_unroll(edpp_ctx* c, int u, const T* in, T* out, const int32_t* idx, int64_t ngrp,
+                            const GsPass& q, bool scatter) {
     switch (u) {
-        case 1: launch_gs_kernel<T, 1, RNT>(c, in, out, idx, ngrp, from, span, scatter); break;
-        case 2: launch_gs_kernel<T, 2, RNT>(c, in, out, idx, ngrp, from, span, scatter); break;
-        case 8: launch_gs_kernel<T, 8, RNT>(c, in, out, idx, ngrp, from, span, scatter); break;
-        default: launch_gs_kernel<T, 4, RNT>(c, in, out, idx, ngrp, from, span, scatter); break;
+        case 1: launch_gs_kernel<T, 1, RNT>(c, in, out, idx, ngrp, q, scatter); break;
+        case 2: launch_gs_kernel<T, 2, RNT>(c, in, out, idx, ngrp, q, scatter); break;
+        case 8: launch_gs_kernel<T, 8, RNT>(c, in, out, idx, ngrp, q, scatter); break;
+        default: launch_gs_kernel<T, 4, RNT>(c, in, out, idx, ngrp, q, scatter); break;
     }
 }
 
@@ -160,16 +270,29 @@ int launch_gather(edpp_ctx* c, const T* in, T* out, const int32_t* idx, int64_t 
     const bool vec = aligned16(in) && aligned16(out) && aligned16(idx);
     const int64_t ngrp = vec ? n / 4 : 0;
     const int64_t len = ceil_div(n, passes);  // gatherImpl.cpp:37
+    uint8_t* done = nullptr;
+    if (ngrp && passes > 1) {
+        const int64_t grid = ceil_div(ngrp, (int64_t)kGsBlock * c->tune.gs_unroll);
+        void* w;
+        int rc = ws_reserve(c, (size_t)grid, &w);
+        if (rc) return rc;
+        done = static_cast<uint8_t*>(w);
+    }
     for (int p = 0; p < passes; ++p) {
         const int64_t lo = (int64_t)p * len;
         if (lo >= n) break;
-        const uint32_t from = (uint32_t)lo;
-        const uint32_t span = (uint32_t)(lo + len <= n ? len : n - lo);
+        GsPass q;
+        q.from = (uint32_t)lo;
+        q.span = (uint32_t)(lo + len <= n ? len : n - lo);
+        q.nall = (uint32_t)n;
+        q.done = done;
+        q.phase = passes == 1 ? kGsOnly : (p == 0 ? kGsFirst : kGsLater);
+        const uint32_t from = q.from, span = q.span;
         if (ngrp) {
             if (c->tune.gs_random_nt > 0)
-                dispatch_unroll<T, true>(c, c->tune.gs_unroll, in, out, idx, ngrp, from, span, scatter);
+                dispatch_unroll<T, true>(c, c->tune.gs_unroll, in, out, idx, ngrp, q, scatter);
             else
-                dispatch_unroll<T, false>(c, c->tune.gs_unroll, in, out, idx, ngrp, from, span, scatter);
+                dispatch_unroll<T, false>(c, c->tune.gs_unroll, in, out, idx, ngrp, q, scatter);
         }
         if (ngrp * 4 < n) {
             int64_t rest = n - ngrp * 4;

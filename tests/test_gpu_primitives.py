@@ -178,6 +178,43 @@ def test_gather_scatter(torch_gpu, ctx, orc, n, kind):
         assert np.array_equal(host(out), sref), ("scatter", passes)
 
 
+@pytest.mark.parametrize("unroll", [1, 2, 4, 8])
+@pytest.mark.parametrize("kind", [0, 1])
+def test_gather_scatter_window(torch_gpu, ctx, orc, unroll, kind):
+    # block-local indices go through the LDS window: whole (4 vectors per
+    # lane: one 4096-element block per workgroup), shared by two or four
+    # workgroups (1, 2), cut by pass slices (3 and 5 passes), or too wide (8)
+    torch = torch_gpu
+    ctx.set_tuning(gs_unroll=unroll)
+    try:
+        for n in (4096 * 9, 4096 * 64 + 100):
+            vals = orc.gen_i32(n, 31 + n)
+            idx = orc.gen_perm(n, 32 + n, kind)
+            dv, di = dev(torch, vals), dev(torch, idx)
+            gref = vals[idx]
+            sref = np.empty_like(vals)
+            sref[idx] = vals
+            out = empty(torch, n, torch.int32)
+            for passes in (1, 3, 5):
+                out.fill_(0)
+                ctx.gather(dv, out, di, n, passes)
+                assert np.array_equal(host(out), gref), ("gather", unroll, n, passes)
+                out.fill_(0)
+                ctx.scatter(dv, out, di, n, passes)
+                assert np.array_equal(host(out), sref), ("scatter", unroll, n, passes)
+            w = orc.gen_i64(n, 33 + n)
+            dw = dev(torch, w)
+            out64 = empty(torch, n, torch.int64)
+            ctx.gather(dw, out64, di, n, 1, wide=True)
+            assert np.array_equal(host(out64), w[idx])
+            ctx.scatter(dw, out64, di, n, 2, wide=True)
+            sw = np.empty_like(w)
+            sw[idx] = w
+            assert np.array_equal(host(out64), sw)
+    finally:
+        ctx.set_tuning()
+
+
 @pytest.mark.parametrize("n", [7, 4096 * 3 + 1, 1 << 20])
 def test_gather_scatter_i64_and_unaligned(torch_gpu, ctx, orc, n):
     torch = torch_gpu
