@@ -26,7 +26,7 @@ typedef struct edpp_tuning {
     int32_t split_tile_mult;  /* split tile >= mult * buckets elements: 16..1024; 0 = default (16)       */
     int32_t scan_block;       /* lanes per look-back workgroup: 256, 512, 1024; 0 = default              */
     int32_t split_block;      /* lanes per WG-split scatter workgroup: 256, or 512 (4-byte keys, values
-                                 staged after keys, 16 keys per lane); 0 = default (512 from 128 buckets) */
+                                 staged after keys, 16 keys per lane); 0 = default (512 from 256 buckets) */
     int32_t reserved[9];
 } edpp_tuning;
 
