@@ -53,7 +53,8 @@ def parse():
     p.add_argument("--passes", type=int, default=0, help="pass count of the default step (0 = library schedule)")
     p.add_argument("--out", default=None, help="also write the JSON line to this file")
     p.add_argument("--profile", default=None,
-                   choices=["stream", "gather", "scatter", "scan", "split", "sort", "join", "batched"],
+                   choices=["stream", "gather", "scatter", "gather_local", "scatter_local", "scan", "split", "sort",
+                            "join", "batched"],
                    help="only run one workload at full size (for rocprofv3), plus one STREAM copy")
     return p.parse_args()
 
@@ -188,10 +189,11 @@ def profile_only(args, ctx, edpp, torch):
     b = torch.empty(n, dtype=i32, device="cuda")
     ctx.gen_i32(a, 0xDA7A)
     reps = max(1, args.steps)
-    if args.profile in ("gather", "scatter"):
+    if args.profile in ("gather", "scatter", "gather_local", "scatter_local"):
         idx = torch.empty(n, dtype=i32, device="cuda")
-        ctx.gen_perm(idx, 0x5EED, edpp.PERM_RANDOM)
-        f = ctx.gather if args.profile == "gather" else ctx.scatter
+        local = args.profile.endswith("_local")
+        ctx.gen_perm(idx, 0x5EED, edpp.PERM_BLOCK_LOCAL if local else edpp.PERM_RANDOM)
+        f = ctx.gather if args.profile.startswith("gather") else ctx.scatter
         for _ in range(reps):
             f(a, b, idx, n, args.passes)
     elif args.profile == "scan":
