@@ -411,6 +411,32 @@ def suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs):
                                    "frac_copy": round(g / copy_gbs, 4)}
             print("split", B, dname, sp[f"B{B}_{dname}"], file=sys.stderr, flush=True)
     res["split_kv"] = sp
+    # the three split strategies (WG, WI, single-lane chunks), B=256 uniform
+    ctx.gen_split_keys(src, b, 256, 0, 0x5917 + 256)
+    ss = {}
+    for sname, strat in [("wg", edpp.SPLIT_WG), ("wi", edpp.SPLIT_WI), ("single", edpp.SPLIT_SINGLE)]:
+        t = T.time(lambda: ctx.split(src, ko, 256, b, vo, start, n, strat), reps=3, warm=1)
+        g = gbs(16.0 * n, t["hampel_ms"])
+        ss[sname] = {"gbs": round(g, 2), "ms": round(t["hampel_ms"], 3), "frac_copy": round(g / copy_gbs, 4)}
+    res["split_strategies_B256"] = ss
+    print("split strategies", ss, file=sys.stderr, flush=True)
+    # local (matrix) scans: one 512-lane workgroup scans 512*TILE int32, 1000
+    # launches back to back, as opencl/test/test_scan_local.cpp:6-137 repeats them
+    ls = {}
+    for tile in (1, 11, 49):
+        m = 512 * tile
+        x = b[:m]
+        ctx.gen_i32(x, 0x10CA1 + tile, 15)
+        for mname, mt in [("lm", edpp.MATRIX_LM), ("reg", edpp.MATRIX_REG), ("lm_reg", edpp.MATRIX_LM_REG),
+                          ("lm_serial", edpp.MATRIX_LM_SERIAL)]:
+            def launches():
+                for _ in range(1000):
+                    ctx.scan_local(x, dst, tile, 512, mt, edpp.NET_WAVE)
+            t = T.time(launches, reps=3, warm=1)
+            us = t["hampel_ms"]  # ms per 1000 launches = us per launch
+            ls[f"tile{tile}_{mname}"] = {"us_per_launch": round(us, 2), "mkeys_s": round(m / us, 1)}
+    res["scan_local_512"] = ls
+    print("scan_local", ls, file=sys.stderr, flush=True)
     # LSD radix sort on the split: 2^28 (int32 key, int32 value) pairs, all 32 key bits
     ctx.gen_i32(src, 0x50F7)
     b.copy_(torch.arange(n, dtype=i32, device="cuda"))
