@@ -41,8 +41,8 @@ struct edpp_ctx {
         edpp_tuning t = {};
         t.gs_unroll = 4;
         t.gs_random_nt = 0;
-        t.scan_vpt = 16;
-        t.scan_block = 512;
+        t.scan_vpt = 8;
+        t.scan_block = 1024;
         t.split_ipt = 16;
         t.split_tile_mult = 16;
         return t;

@@ -23,7 +23,8 @@ struct TileScan {
     static_assert(BLOCK % kWave == 0, "block must be whole waves");
 
     struct Smem {
-        T wave_tot[VPT][NW];
+        T wave_tot[VPT][NW];  // wave totals, then their exclusive offsets in tile order
+        T total;
         T prefix;
         uint32_t tile;
     };
@@ -32,6 +33,10 @@ struct TileScan {
     T off[VPT];   // exclusive offset of the lane's first element of sub-tile j, within the tile
     T total;      // tile sum
 
+    // The last, partial tile takes the same 16-byte vectors wherever a whole
+    // vector lies below n; only the one vector that straddles n goes element
+    // by element.  (A fully element-wise tail path holds 64-bit addresses for
+    // every element at once and sizes the registers of the whole kernel.)
     __device__ __forceinline__ void load(const T* __restrict__ in, int64_t base, int64_t n) {
         const int t = threadIdx.x;
         if (base + TILE <= n) {
@@ -44,10 +49,14 @@ struct TileScan {
         } else {
             #pragma unroll
             for (int j = 0; j < VPT; ++j) {
-                #pragma unroll
-                for (int k = 0; k < EPV; ++k) {
-                    int64_t e = base + (int64_t)j * SUB + (int64_t)t * EPV + k;
-                    v[j][k] = e < n ? in[e] : T(0);
+                const int64_t e0 = base + (int64_t)j * SUB + (int64_t)t * EPV;
+                if (e0 + EPV <= n) {
+                    V x = ld_stream(reinterpret_cast<const V*>(in + e0));
+                    #pragma unroll
+                    for (int k = 0; k < EPV; ++k) v[j][k] = x[k];
+                } else {
+                    #pragma unroll
+                    for (int k = 0; k < EPV; ++k) v[j][k] = e0 + k < n ? in[e0 + k] : T(0);
                 }
             }
         }
@@ -79,9 +88,11 @@ struct TileScan {
         }
     }
 
-    // Block-wide scan of the tile; contains one __syncthreads.  off[j] first
-    // holds the lane's exclusive prefix inside its wave, then gets the
-    // earlier waves and sub-tiles added (one array of VPT values per lane).
+    // Block-wide scan of the tile; contains two __syncthreads.  off[j] first
+    // holds the lane's exclusive prefix inside its wave; wave 0 then turns the
+    // VPT*NW wave totals into exclusive offsets in tile order (sub-tile, then
+    // wave), and each lane adds the one offset of its wave to each off[j].
+    // (Every lane summing all wave totals itself holds them all in registers.)
     __device__ __forceinline__ void scan(Smem& sm) {
         const int lane = lane_id();
         const int w = threadIdx.x / kWave;
@@ -95,20 +106,32 @@ struct TileScan {
             if (lane == kWave - 1) sm.wave_tot[j][w] = inc;
         }
         __syncthreads();
-        T run = 0;
-        #pragma unroll
-        for (int j = 0; j < VPT; ++j) {
-            T before = 0, sub = 0;
+        if (w == 0) {
+            constexpr int M = VPT * NW;
+            constexpr int PER = (M + kWave - 1) / kWave;  // consecutive entries per lane
+            T* flat = &sm.wave_tot[0][0];
+            T x[PER];
+            T s = 0;
             #pragma unroll
-            for (int q = 0; q < NW; ++q) {
-                T x = sm.wave_tot[j][q];
-                before += (q < w) ? x : T(0);
-                sub += x;
+            for (int i = 0; i < PER; ++i) {
+                const int idx = lane * PER + i;
+                x[i] = idx < M ? flat[idx] : T(0);
+                s += x[i];
             }
-            off[j] += run + before;
-            run += sub;
+            const T inc = wave_inclusive_add(s);
+            T e = inc - s;
+            #pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                const int idx = lane * PER + i;
+                if (idx < M) flat[idx] = e;
+                e += x[i];
+            }
+            if (lane == kWave - 1) sm.total = inc;
         }
-        total = run;
+        __syncthreads();
+        #pragma unroll
+        for (int j = 0; j < VPT; ++j) off[j] += sm.wave_tot[j][w];
+        total = sm.total;
     }
 
     __device__ __forceinline__ void store(T* __restrict__ out, int64_t base, int64_t n, T prefix) const {
@@ -120,15 +143,16 @@ struct TileScan {
             V o;
             #pragma unroll
             for (int k = 0; k < EPV; ++k) { o[k] = x; x += v[j][k]; }
-            if (full) {
-                st_stream(o, reinterpret_cast<V*>(out + base + (int64_t)j * SUB) + t);
+            const int64_t e0 = base + (int64_t)j * SUB + (int64_t)t * EPV;
+            if (full || e0 + EPV <= n) {
+                st_stream(o, reinterpret_cast<V*>(out + e0));
             } else {
                 #pragma unroll
-                for (int k = 0; k < EPV; ++k) {
-                    int64_t e = base + (int64_t)j * SUB + (int64_t)t * EPV + k;
-                    if (e < n) out[e] = o[k];
-                }
+                for (int k = 0; k < EPV; ++k)
+                    if (e0 + k < n) out[e0 + k] = o[k];
             }
+            // one sub-tile's outputs at a time: its inputs die as its store issues
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 };

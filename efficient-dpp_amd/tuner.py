@@ -17,7 +17,7 @@ import numpy as np
 
 OPS = ("scan", "gather", "scatter", "split")
 FAMILY_OF_OP = {0: 0, 1: 1, 2: 1, 3: 2}
-# one fixed schedule per family: the method without the model (look-back 512x16,
+# one fixed schedule per family: the method without the model (look-back 1024x8,
 # one pass, WG split with 16 keys per lane and tiles of 64 elements per bucket)
 DEFAULT_CANDIDATE = {0: 0, 1: 5, 2: 11}
 
