@@ -587,14 +587,20 @@ def learned(ctx, edpp, torch, args):
     ctx.set_model(None)
     e2e_default = tn.end_to_end(ws, ctx, specs, False)
     ctx.set_model(mlp)
+    mlp.memo(False)  # every call runs the selection kernel and waits for it
+    e2e_model_cold = tn.end_to_end(ws, ctx, specs, True)
+    mlp.memo(True)  # repeated calls reuse the model's decision
     e2e_model = tn.end_to_end(ws, ctx, specs, True)
     ctx.set_model(None)
+    auto = tn.regret(test_recs, tn.auto_choices(test_recs))
     best_bytes = sum(tn.call_bytes(s) for s in specs)
     out = {"train_calls": len(train_recs), "test_calls": len(test_recs), "autotune_s": round(t1 - t0, 1),
            "train_s": round(t2 - t1, 2), "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
-           "regret_model": model, "regret_default": default, "regret_model_on_train": train_fit,
+           "regret_model": model, "regret_default": default, "regret_builtin_schedule": auto,
+           "regret_model_on_train": train_fit,
            "exhaustive_gbs": round(best_bytes / (model["ms_best"] * 1e-3) / 1e9, 1),
-           "end_to_end_model": e2e_model, "end_to_end_default": e2e_default,
+           "end_to_end_model": e2e_model, "end_to_end_model_every_call_selects": e2e_model_cold,
+           "end_to_end_default": e2e_default,
            "best_candidate_histogram": tn.summarize_times(train_recs)}
     print("learned", json.dumps(out), file=sys.stderr, flush=True)
     mlp.close()

@@ -110,6 +110,8 @@ _SIGS["edpp_candidate_info"] = [c_i32, ctypes.POINTER(c_i32), ctypes.c_char_p, c
 _SIGS["edpp_call_run"] = [c_vp, ctypes.POINTER(Call), c_i32, c_dp]
 _SIGS["edpp_autotune"] = [c_vp, ctypes.POINTER(Call), c_i32, ctypes.POINTER(ctypes.c_double)]
 _SIGS["edpp_call_features"] = [c_vp, ctypes.POINTER(Call), c_vp]
+_SIGS["edpp_default_candidate"] = [c_vp, ctypes.POINTER(Call), ctypes.POINTER(c_i32)]
+_SIGS["edpp_model_choice"] = [c_vp, ctypes.POINTER(Call), ctypes.POINTER(c_i32)]
 _SIGS["edpp_mlp_num_params"] = []
 _SIGS["edpp_mlp_create"] = [c_vp, c_u64, ctypes.POINTER(c_vp)]
 _SIGS["edpp_mlp_destroy"] = [c_vp]
@@ -119,6 +121,7 @@ _SIGS["edpp_mlp_forward"] = [c_vp, c_vp, c_vp, c_i32, c_vp, c_vp]
 _SIGS["edpp_mlp_loss_grad"] = [c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp]
 _SIGS["edpp_mlp_train"] = [c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, ctypes.c_float, _c_fp]
 _SIGS["edpp_ctx_set_model"] = [c_vp, c_vp]
+_SIGS["edpp_mlp_memo"] = [c_vp, c_i32]
 
 _RESTYPES = {"edpp_strerror": ctypes.c_char_p, "edpp_last_error": ctypes.c_char_p, "edpp_ctx_stream": c_vp}
 
@@ -236,6 +239,18 @@ class Context:
         out = (ctypes.c_double * MAX_CANDIDATES)()
         self._check(self._L.edpp_autotune(self.h, ctypes.byref(call), int(reps), out))
         return list(out)
+
+    def default_candidate(self, call):
+        """Candidate index of the built-in schedule for this call (-1: none)."""
+        k = c_i32(-1)
+        self._check(self._L.edpp_default_candidate(self.h, ctypes.byref(call), ctypes.byref(k)))
+        return k.value
+
+    def model_choice(self, call):
+        """Candidate index the context's model picks for this call."""
+        k = c_i32(-1)
+        self._check(self._L.edpp_model_choice(self.h, ctypes.byref(call), ctypes.byref(k)))
+        return k.value
 
     def call_features(self, call, d_features):
         self._check(self._L.edpp_call_features(self.h, ctypes.byref(call), _ptr(d_features)))
@@ -431,6 +446,11 @@ class MLP:
         self.ctx._check(self.ctx._L.edpp_mlp_train(self.h, _ptr(x), _ptr(target), _ptr(mask), x.shape[0],
                                                    int(epochs), float(lr), losses.ctypes.data_as(_c_fp)))
         return losses[:epochs]
+
+    def memo(self, enabled=True):
+        """Remember the model's choice per AUTO call signature (on by default);
+        either setting forgets what was remembered."""
+        self.ctx._check(self.ctx._L.edpp_mlp_memo(self.h, 1 if enabled else 0))
 
     def close(self):
         if getattr(self, "h", None):

@@ -158,11 +158,18 @@ int default_split_strategy(edpp_ctx* c, int64_t n, int buckets) {
     (void)c; (void)n; (void)buckets;
     return EDPP_SPLIT_WG;
 }
-void default_split_shape(edpp_ctx* c, int buckets, int key_bytes, int* ipt, int* mult) {
+// Keys per lane of the WG split, from an autotuner sweep of 2^16..2^28
+// pairs: 8 below 2^22 (more workgroups while the call is latency-bound),
+// and for up to 16 buckets up to 2^24 and for two-pass bucket counts up to
+// 2^26; 16 otherwise (from 256 buckets the 512-lane scatter needs it: B=256
+// at 2^28 is 1.33 ms at 16 against 2.31 ms at 8).
+void default_split_shape(edpp_ctx* c, int64_t n, int buckets, int key_bytes, int* ipt, int* mult) {
     (void)c;
-    (void)buckets;
     (void)key_bytes;
-    *ipt = 16;   // wg_ipt() falls back to 8 when the LDS reorder buffer would not fit
+    int lg = 0;
+    while ((1 << lg) < buckets) ++lg;
+    const bool small = n < (1ll << 22) || (lg <= 4 && n <= (1ll << 24)) || (lg > 8 && n <= (1ll << 26));
+    *ipt = small ? 8 : 16;  // wg_ipt() also falls back to 8 when the LDS reorder buffer would not fit
     *mult = 16;
 }
 

@@ -109,6 +109,6 @@ int default_passes(edpp_ctx* c, int64_t n, int elem_bytes, bool scatter);
 int default_scan_scheme(edpp_ctx* c, int64_t n, int elem_bytes);
 void default_scan_shape(edpp_ctx* c, int64_t n, int* block, int* vpt);
 int default_split_strategy(edpp_ctx* c, int64_t n, int buckets);
-void default_split_shape(edpp_ctx* c, int buckets, int key_bytes, int* ipt, int* mult);
+void default_split_shape(edpp_ctx* c, int64_t n, int buckets, int key_bytes, int* ipt, int* mult);
 
 }  // namespace edpp

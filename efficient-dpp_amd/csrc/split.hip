@@ -446,7 +446,7 @@ int launch_split(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t
     }
     if (strategy == EDPP_SPLIT_AUTO) {
         const edpp_tuning saved = c->tune;
-        default_split_shape(c, buckets, (int)sizeof(K), &c->tune.split_ipt, &c->tune.split_tile_mult);
+        default_split_shape(c, n, buckets, (int)sizeof(K), &c->tune.split_ipt, &c->tune.split_tile_mult);
         int rc = launch_split<K>(c, kin, kout, vin, vout, d_start, n, buckets, default_split_strategy(c, n, buckets));
         c->tune = saved;
         return rc;
@@ -506,7 +506,7 @@ int launch_sort_u32(edpp_ctx* c, const uint32_t* kin, uint32_t* kout, const int3
     const int passes = (bits + kMaxSplitBitsPerPass - 1) / kMaxSplitBitsPerPass;
     const int per = (bits + passes - 1) / passes;  // balanced digits of at most 8 bits
     const edpp_tuning saved = c->tune;
-    default_split_shape(c, 1 << per, 4, &c->tune.split_ipt, &c->tune.split_tile_mult);
+    default_split_shape(c, n, 1 << per, 4, &c->tune.split_ipt, &c->tune.split_tile_mult);
     int64_t hlen = 0;
     for (int p = 0; p < passes; ++p) {
         const int lo = begin_bit + p * per, hi = std::min(end_bit, lo + per);

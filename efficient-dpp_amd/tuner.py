@@ -90,7 +90,8 @@ def measure(ws, specs, reps=3, log=None):
     for i, spec in enumerate(specs):
         call = ws.prepare(spec)
         times = ws.ctx.autotune(call, reps)
-        out.append({"spec": spec, "times": times, "features": ws.features(call)})
+        out.append({"spec": spec, "times": times, "features": ws.features(call),
+                    "auto": ws.ctx.default_candidate(call)})
         if log and (i + 1) % 100 == 0:
             log(f"autotuned {i + 1}/{len(specs)} calls")
     return out
@@ -153,9 +154,18 @@ def default_choices(records):
     return [DEFAULT_CANDIDATE[FAMILY_OF_OP[r["spec"]["op"]]] for r in records]
 
 
+def auto_choices(records):
+    """The built-in schedule's candidate for every record (the method without
+    the model); a record whose built-in schedule is no candidate falls back to
+    the family's fixed default."""
+    return [r["auto"] if r.get("auto", -1) >= 0 else DEFAULT_CANDIDATE[FAMILY_OF_OP[r["spec"]["op"]]]
+            for r in records]
+
+
 def end_to_end(ws, ctx, specs, use_model, reps=3):
     """Sum of the measured time of the calls run with cand = -1 (the context's
-    schedule choice, including the model's feature and inference kernel)."""
+    schedule choice, including the model's feature and inference kernel when
+    the model's decision memo does not already hold the call)."""
     total_ms, total_bytes = 0.0, 0.0
     for spec in specs:
         call = ws.prepare(spec)

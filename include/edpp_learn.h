@@ -63,6 +63,9 @@ int edpp_call_run(edpp_ctx* ctx, const edpp_call* call, int32_t cand, double* ms
  * timed runs after one warm-up, median); times[k] = ms, or -1 for candidates
  * of other families.  times has EDPP_MAX_CANDIDATES entries. */
 int edpp_autotune(edpp_ctx* ctx, const edpp_call* call, int32_t reps, double* times);
+/* The candidate the built-in schedule (no model) gives this call, or -1 when
+ * that schedule is none of the candidates. */
+int edpp_default_candidate(edpp_ctx* ctx, const edpp_call* call, int32_t* cand);
 /* Features of a call, computed on the device from samples of its inputs. */
 int edpp_call_features(edpp_ctx* ctx, const edpp_call* call, float* d_features);
 
@@ -85,6 +88,20 @@ int edpp_mlp_train(edpp_mlp* m, const float* d_x, const float* d_target, const f
                    int32_t epochs, float lr, float* loss_host);
 /* Let AUTO calls on ctx pick their schedule with the model (NULL: built-in defaults). */
 int edpp_ctx_set_model(edpp_ctx* ctx, edpp_mlp* m);
+/* Decision memo of AUTO calls (no counterpart in the reference, like the rest
+ * of this header).  On (the default), the model remembers its choice per call
+ * signature: op, element width, n, buckets, whether values are given, and the
+ * input and index pointers.  A repeated call then skips the selection kernel
+ * and its synchronisation.  Every remembered choice is dropped when the
+ * parameters change (edpp_mlp_train, edpp_mlp_set_params).  A buffer whose
+ * contents change under the same signature keeps its first choice; that can
+ * cost time, never correctness, since every candidate is bit-exact.
+ * enabled = 0 turns the memo off (every AUTO call runs the selection kernel);
+ * either setting forgets the remembered choices. */
+int edpp_mlp_memo(edpp_mlp* m, int32_t enabled);
+/* The candidate the context's model picks for the call (through the memo), as
+ * an AUTO call would; EDPP_EINVAL when no model is set. */
+int edpp_model_choice(edpp_ctx* ctx, const edpp_call* call, int32_t* cand);
 
 #ifdef __cplusplus
 }

@@ -144,3 +144,61 @@ def test_model_gain_and_model_driven_calls(torch_gpu, ctx, edpp, orc):
     finally:
         ctx.set_model(None)
     mlp.close()
+
+
+def test_default_candidate_is_the_builtin_schedule(torch_gpu, ctx, edpp):
+    torch = torch_gpu
+    tn = importlib.import_module("efficient-dpp_amd.tuner")
+    ws = tn.Workspace(edpp, ctx, torch, 1 << 25)
+    names = [c[1] for c in edpp.candidates()]
+    want = {
+        (0, 1 << 16, 4, 256): "scan look-back 1024x4",
+        (0, 1 << 22, 8, 256): "scan look-back 1024x8",
+        (1, 1 << 20, 4, 256): "gather/scatter 1 pass, unroll 4",
+        (2, 1 << 25, 4, 256): "gather/scatter 2 passes",
+        (3, 1 << 20, 4, 256): "split WG 8/lane, tile 16B",
+        (3, 1 << 25, 4, 256): "split WG 16/lane, tile 16B",
+        (3, 1 << 24, 4, 16): "split WG 8/lane, tile 16B",
+    }
+    for (op, n, width, B), name in want.items():
+        spec = {"op": op, "n": n, "width": width, "seed": 3, "buckets": B, "dist": 0, "perm": 0}
+        assert names[ctx.default_candidate(ws.prepare(spec))] == name, (op, n, width, B)
+
+
+def test_model_memo_returns_the_fresh_choice(torch_gpu, ctx, edpp, orc):
+    # remembered choices equal fresh ones, follow parameter changes, survive
+    # eviction (more signatures than slots), and AUTO calls stay bit-exact
+    torch = torch_gpu
+    tn = importlib.import_module("efficient-dpp_amd.tuner")
+    ws = tn.Workspace(edpp, ctx, torch, 1 << 19)
+    specs = tn.sample_specs(80, seed=31, lo=10, hi=19)
+    mlp = edpp.MLP(ctx, seed=7)
+    ctx.set_model(mlp)
+    try:
+        def choices():
+            return [ctx.model_choice(ws.prepare(s)) for s in specs]
+        mlp.memo(False)
+        fresh = choices()
+        mlp.memo(True)
+        assert choices() == fresh  # filled
+        assert choices() == fresh  # 80 signatures through 64 slots: evicted ones are recomputed
+        p = mlp.get_params()
+        q = (p + np.random.default_rng(5).normal(0, 0.5, p.shape)).astype(np.float32)
+        mlp.set_params(q)
+        mlp.memo(False)
+        fresh_q = choices()
+        assert fresh_q != fresh  # the perturbation changes some choice, so a stale memo would show
+        mlp.memo(True)
+        mlp.set_params(p)
+        assert choices() == fresh  # remembered under p
+        mlp.set_params(q)
+        assert choices() == fresh_q  # the change of parameters dropped the choices made under p
+        for spec in specs[:9]:
+            call = ws.prepare(spec)
+            for _ in range(2):  # the second call is served by the memo
+                ws.b.fill_(-1)
+                ctx.call_run(call, -1)
+                assert _oracle_check(orc, ws, spec), spec
+    finally:
+        ctx.set_model(None)
+    mlp.close()
