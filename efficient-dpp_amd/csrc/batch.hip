@@ -45,10 +45,14 @@ struct Seg {
     int32_t job, phase;
 };
 
+// keys per lane of a split tile: 8 keeps the persistent kernel at about 100
+// registers (16 for 32-bit keys needed 200, two workgroups per CU)
+constexpr int kBatchSplitIpt = 8;
+
 // tile lengths per kind and width
 __host__ __device__ inline int64_t tile_elems(int kind, int wide) {
     if (kind == EDPP_JOB_SCAN) return wide ? 2048 : 8192;  // TileScan<u64,256,4>, TileScan<u32,256,8>
-    if (kind == EDPP_JOB_SPLIT) return wide ? 2048 : 4096; // 256 lanes x IPT (8 or 16)
+    if (kind == EDPP_JOB_SPLIT) return 256 * kBatchSplitIpt;  // 256 lanes x kBatchSplitIpt keys
     return 4096;                                           // gather: 256 lanes x 16
 }
 constexpr int64_t kHistScanTile = 4096;  // TileScan<uint32_t,256,4>
@@ -244,8 +248,8 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const DJob* __restrict__ 
         const int64_t t = tk - sg.first;
         uint32_t* lds = reinterpret_cast<uint32_t*>(smem);
         if (sg.phase == kPhaseHist) {
-            if (J.wide) split_hist_tile<uint64_t, 8>(J, t, lds);
-            else split_hist_tile<uint32_t, 16>(J, t, lds);
+            if (J.wide) split_hist_tile<uint64_t, kBatchSplitIpt>(J, t, lds);
+            else split_hist_tile<uint32_t, kBatchSplitIpt>(J, t, lds);
         } else if (sg.phase == kPhaseHistScan) {
             wait_count(J.ctr + 0, (uint32_t)J.tiles, spin_limit, err, &s_ok);
             LookbackState32 st;
@@ -257,8 +261,8 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const DJob* __restrict__ 
             signal_count(J.ctr + 1);
         } else if (sg.phase == kPhaseScatter) {
             wait_count(J.ctr + 1, (uint32_t)J.hscan_tiles, spin_limit, err, &s_ok);
-            if (J.wide) split_scatter_tile<uint64_t, 8>(J, t, lds);
-            else split_scatter_tile<uint32_t, 16>(J, t, lds);
+            if (J.wide) split_scatter_tile<uint64_t, kBatchSplitIpt>(J, t, lds);
+            else split_scatter_tile<uint32_t, kBatchSplitIpt>(J, t, lds);
         } else if (J.kind == EDPP_JOB_SCAN) {
             if (J.wide) {
                 LookbackState64 st;
@@ -281,7 +285,11 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const DJob* __restrict__ 
     }
 }
 
-constexpr size_t kBatchLds = 48 * 1024;  // split scatter, 32-bit keys: 8192 + 7232 + 4096*8 bytes
+// LDS of the largest tile: the split scatter with 64-bit keys at 256 buckets
+// (ranking masks, counts and offsets, then 2048 keys and 2048 values): 40000
+// bytes, so four workgroups share a CU
+constexpr size_t kBatchLds =
+    (size_t)kbuf_word_offset(256) * 4 + (size_t)kBatchBlock * kBatchSplitIpt * (sizeof(uint64_t) + sizeof(int32_t));
 
 }  // namespace
 }  // namespace edpp

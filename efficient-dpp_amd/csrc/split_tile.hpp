@@ -57,8 +57,8 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t g, int64_t tiles, int xcds) 
 // whist[waves][B], run[B], sub[B], off[B], wtmp[16], then the key and value
 // reorder buffers from a 16-byte boundary (the value buffer aliases the key
 // buffer when values are staged after keys)
-__host__ __device__ __forceinline__ int whist_word_offset(int B, int waves = kSplitWaves) { return 2 * waves * B; }
-__host__ __device__ __forceinline__ int kbuf_word_offset(int B, int waves = kSplitWaves) {
+__host__ __device__ constexpr int whist_word_offset(int B, int waves = kSplitWaves) { return 2 * waves * B; }
+__host__ __device__ constexpr int kbuf_word_offset(int B, int waves = kSplitWaves) {
     return ((whist_word_offset(B, waves) + (waves + 3) * B + 16) + 3) & ~3;
 }
 
