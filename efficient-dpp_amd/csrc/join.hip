@@ -23,7 +23,7 @@ namespace {
 
 constexpr int kJoinBlock = 1024;
 constexpr int kJoinSlots = 16384;  // 64-bit LDS slots (128 KiB): (count << 32) | key; count 0 = empty
-constexpr int kJoinChunk = 8192;   // build keys per table fill
+constexpr int kJoinChunk = kJoinSlots / 2;  // build keys per table fill
 constexpr int kJoinTargetBuild = 4096;
 
 // murmur3's 32-bit finalizer: a bijection of 32-bit words
