@@ -368,12 +368,12 @@ def test_split_full_size_properties(torch_gpu, ctx, B):
 # ---------------------------------------------------------------- radix sort
 @pytest.mark.parametrize("signed", [False, True])
 @pytest.mark.parametrize("bits", [(0, 32), (0, 16), (4, 20), (8, 32), (0, 8), (3, 4), (7, 7)])
-@pytest.mark.parametrize("n", [0, 1, 1000, 100_003, (1 << 20) + 17])
+@pytest.mark.parametrize("n", [0, 1, 1000, 100_003, (1 << 20) - 1, (1 << 20) + 17, 3_000_001])
 def test_sort(torch_gpu, ctx, orc, signed, bits, n):
     torch = torch_gpu
     keys = orc.gen_i32(n, 500 + n + bits[0])
     if n > 10:
-        keys[: n // 10] = keys[n // 10: n // 5]  # duplicates: stability observable
+        keys[: n // 10] = keys[n // 10: 2 * (n // 10)]  # duplicates: stability observable
     keys = keys if signed else keys.view(np.uint32)
     vals = np.arange(n, dtype=np.int32)
     ko, vo = orc.sort(keys, vals, bits[0], bits[1], signed)
@@ -390,6 +390,28 @@ def test_sort(torch_gpu, ctx, orc, signed, bits, n):
     vin = dev(torch, vals)
     ctx.sort(dk, dk, vin, vin, n, bits[0], bits[1], signed)
     assert np.array_equal(host(dk), ko.view(np.int32)) and np.array_equal(host(vin), vo), ("in place", bits, n)
+
+
+@pytest.mark.parametrize("kind", ["equal", "two", "zipf"])
+def test_sort_skewed_keys(torch_gpu, ctx, orc, kind):
+    # one-sweep passes with every key of a tile in one bucket (equal), two
+    # values (0 and -1: the flipped top digit orders -1 first), Zipf low bytes
+    torch = torch_gpu
+    n = 3_000_001
+    if kind == "equal":
+        keys = np.full(n, 0x12345678, np.int32)
+    elif kind == "two":
+        keys = np.where(orc.gen_i32(n, 41) & 1, -1, 0).astype(np.int32)
+    else:
+        keys, _ = orc.gen_split_keys(n, 256, 1, 42)
+    vals = np.arange(n, dtype=np.int32)
+    for signed in (False, True):
+        ko, vo = orc.sort(keys if signed else keys.view(np.uint32), vals, 0, 32, signed)
+        kout = empty(torch, n, torch.int32)
+        vout = empty(torch, n, torch.int32)
+        ctx.sort(dev(torch, keys), kout, dev(torch, vals), vout, n, 0, 32, signed)
+        assert np.array_equal(host(kout), ko.view(np.int32)), (kind, signed)
+        assert np.array_equal(host(vout), vo), (kind, signed)
 
 
 def test_sort_full_size_properties(torch_gpu, ctx):
