@@ -188,7 +188,45 @@ __device__ __forceinline__ void count_tiles(const K* __restrict__ kin, int64_t s
     #pragma unroll
     for (int t = 0; t < TPW; ++t) {
         uint32_t* row = whist + ((t / spt) * kSplitWaves + w) * B;
-        if constexpr (REGW > 0) {
+        if constexpr (REGW >= 2) {
+            // 8 or 16 buckets: 4-bit counters, bucket b at bits [4b, 4b+4) of one
+            // 64-bit word per item parity (at most IPT/2 <= 15 items each), so a
+            // key costs a shift and an add instead of a compare-and-select per
+            // packed word (a wave64 instruction holds a SIMD for 4 cycles)
+            static_assert(IPT / 2 <= 15, "4-bit counters");
+            uint64_t acc0 = 0, acc1 = 0;
+            #pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int64_t e = wbase + (int64_t)t * S + (int64_t)j * kWave;
+                const uint32_t b = bucket_of(key[t][j], shift, mask, flip);
+                const uint64_t inc = e < n ? 1ull << (4 * b) : 0ull;
+                if (j & 1) acc1 += inc;
+                else acc0 += inc;
+            }
+            constexpr uint64_t kNib = 0x0F0F0F0F0F0F0F0Full;
+            const uint64_t ev = (acc0 & kNib) + (acc1 & kNib);                // byte k: bucket 2k
+            const uint64_t od = ((acc0 >> 4) & kNib) + ((acc1 >> 4) & kNib);  // byte k: bucket 2k+1
+            #pragma unroll
+            for (int h = 0; h < REGW / 2; ++h) {  // buckets 8h .. 8h+7
+                const uint32_t we = (uint32_t)(ev >> (32 * h)), wo = (uint32_t)(od >> (32 * h));
+                // two 16-bit fields per sum: at most 64*IPT per bucket
+                const uint32_t e04 = wave_sum(we & 0x00FF00FFu);         // buckets 8h, 8h+4
+                const uint32_t e26 = wave_sum((we >> 8) & 0x00FF00FFu);  // buckets 8h+2, 8h+6
+                const uint32_t o15 = wave_sum(wo & 0x00FF00FFu);         // buckets 8h+1, 8h+5
+                const uint32_t o37 = wave_sum((wo >> 8) & 0x00FF00FFu);  // buckets 8h+3, 8h+7
+                if (lane == 0) {  // the row belongs to this wave alone
+                    const int b0 = 8 * h;
+                    row[b0 + 0] += e04 & 0xFFFFu;
+                    row[b0 + 4] += e04 >> 16;
+                    row[b0 + 2] += e26 & 0xFFFFu;
+                    row[b0 + 6] += e26 >> 16;
+                    row[b0 + 1] += o15 & 0xFFFFu;
+                    row[b0 + 5] += o15 >> 16;
+                    row[b0 + 3] += o37 & 0xFFFFu;
+                    row[b0 + 7] += o37 >> 16;
+                }
+            }
+        } else if constexpr (REGW > 0) {
             uint32_t packed[REGW];  // bucket b: word b/4, bits [8(b%4), 8(b%4)+8)
             #pragma unroll
             for (int q = 0; q < REGW; ++q) packed[q] = 0;

@@ -277,7 +277,7 @@ SPLIT_SIZES = [0, 1, 63, 100, 4096 * 3 + 5, 100_003, 1 << 20]
 
 
 @pytest.mark.parametrize("strategy", [1, 2, 3])
-@pytest.mark.parametrize("B", [1, 2, 16, 256, 4096])
+@pytest.mark.parametrize("B", [1, 2, 4, 8, 16, 32, 256, 4096])
 @pytest.mark.parametrize("n", SPLIT_SIZES)
 def test_split_i32(torch_gpu, ctx, orc, strategy, B, n):
     torch = torch_gpu
@@ -300,7 +300,7 @@ def test_split_i32(torch_gpu, ctx, orc, strategy, B, n):
 
 @pytest.mark.parametrize("ipt,mult,block", [(8, 16, 0), (8, 64, 0), (16, 16, 0), (16, 64, 0), (16, 256, 0),
                                             (16, 16, 256), (16, 16, 512), (16, 64, 512), (16, 96, 512)])
-@pytest.mark.parametrize("B", [2, 16, 128, 256, 4096])
+@pytest.mark.parametrize("B", [2, 8, 16, 128, 256, 4096])
 def test_split_wg_shapes(torch_gpu, ctx, orc, ipt, mult, block, B):
     # sub-tile shapes of the WG strategy (256- and 512-lane scatter workgroups,
     # tiles of 1..several sub-tiles), sizes ragged against both the sub-tile
