@@ -75,13 +75,18 @@ class Timer:
     def __init__(self, torch):
         self.torch = torch
 
-    def time(self, fn, reps, warm=2):
+    def time(self, fn, reps, warm=3, flush=None):
+        """`warm` untimed runs, then `reps` runs each bracketed by events.
+        flush, when given, runs before each timed run outside the events (a
+        1 GiB scratch write that evicts the 256 MiB Infinity Cache: cold runs)."""
         torch = self.torch
         for _ in range(warm):
             fn()
         evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
         torch.cuda.synchronize()
         for a, b in evs:
+            if flush is not None:
+                flush()
             a.record()
             fn()
             b.record()
@@ -121,12 +126,14 @@ def cpu_baseline(args, torch_unused=None):
             break
     ms = hampel(times)
     cpu = platform.processor() or platform.machine()
+    sockets = set()
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
+                if line.startswith("model name") and cpu == (platform.processor() or platform.machine()):
                     cpu = line.split(":", 1)[1].strip()
-                    break
+                elif line.startswith("physical id"):
+                    sockets.add(line.split(":", 1)[1].strip())
     except OSError:
         pass
     return {
@@ -136,8 +143,56 @@ def cpu_baseline(args, torch_unused=None):
         "kind": kind,
         "sample": f"one OpenMP gather + one scatter of 2^{int(np.log2(n))} int32 through a random permutation, "
                   f"Hampel mean of {len(times)} runs ({ms:.1f} ms); host {cpu}, nproc={os.cpu_count()}, "
-                  f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')}",
+                  f"sockets={len(sockets) or 'unknown'}, OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')}, "
+                  f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND', 'unset')}",
+        "primitives": cpu_primitives(orc, ref, b, 1 << 25),
     }
+
+
+def cpu_primitives(orc, ref, b, n):
+    """The other host primitives of the CPU baseline on n int32: the scans
+    (scan-scan-add and reduce-then-scan, openmp/test_scan_CPU.cpp:52-163,
+    restated: that file needs oneTBB), copy and scale
+    (openmp/test_bandwidth_CPU.cpp:39-83, the reference's own when built),
+    gather and scatter through a random permutation.  Run 0 is first touch
+    and discarded, then 5 timed runs; GB/s from the Hampel mean."""
+    src = orc.gen_i32(n, 0x5CA7) & 15
+    out = np.empty(n, np.int32)
+    idx = orc.gen_perm(n, 0xC0FFEE, 0)
+    A = orc._a
+    if ref is not None:
+        copy = lambda: ref.edpp_ref_copy(A(src), A(out), n)
+        scale = lambda: ref.edpp_ref_scale(A(src), A(out), n)
+        gather = lambda: ref.edpp_ref_gather(A(src), A(out), A(idx), n)
+        scatter = lambda: ref.edpp_ref_scatter(A(src), A(out), A(idx), n)
+    else:
+        copy = lambda: b.baseline_copy(A(src), A(out), n)
+        scale = lambda: b.baseline_scale(A(src), A(out), n)
+        gather = lambda: b.baseline_gather(A(src), A(out), A(idx), n)
+        scatter = lambda: b.baseline_scatter(A(src), A(out), A(idx), n)
+    res = {}
+    for name, fn, nb, kind in [("scan_ssa", lambda: b.baseline_scan_ssa(A(src), A(out), n), 8, "port"),
+                               ("scan_rts", lambda: b.baseline_scan_rts(A(src), A(out), n), 8, "port"),
+                               ("copy", copy, 8, "reference" if ref is not None else "port"),
+                               ("scale", scale, 8, "reference" if ref is not None else "port"),
+                               ("gather", gather, 12, "reference" if ref is not None else "port"),
+                               ("scatter", scatter, 12, "reference" if ref is not None else "port")]:
+        fn()
+        ts = [fn() for _ in range(5)]
+        res[name] = {"gbs": round(gbs(nb * n, hampel(ts)), 3), "median_ms": round(float(np.median(ts)), 3),
+                     "kind": kind}
+        if name.startswith("scan"):  # exclusive prefix, spot-checked at both ends
+            head = np.concatenate(([0], np.cumsum(src[:4095], dtype=np.int64))).astype(np.int32)
+            tail = int(np.sum(src[:-1], dtype=np.int64) & 0xFFFFFFFF)
+            res[name]["checked"] = bool(np.array_equal(out[:4096], head)) and int(out[-1]) & 0xFFFFFFFF == tail
+    res["n"] = n
+    if os.path.exists("/usr/include/tbb/task_scheduler_init.h"):
+        res["tbb_scan"] = "not built: the TBB headers are present but this baseline does not link TBB"
+    elif os.path.isdir("/usr/include/tbb") or os.path.isdir("/usr/include/oneapi/tbb"):
+        res["tbb_scan"] = "unavailable: the reference's scan_tbb needs tbb/task_scheduler_init.h, absent from oneTBB"
+    else:
+        res["tbb_scan"] = "unavailable: TBB is not installed (the reference's scan_tbb)"
+    return res
 
 
 def pmc_traffic(kernel_prefix, profile, passes, timeout_s=150):
@@ -305,7 +360,7 @@ def main():
     g_passes = passes if passes > 0 else ctx.schedule_passes(n, scatter=False)
     s_passes = passes if passes > 0 else ctx.schedule_passes(n, scatter=True)
     achieved = gbs(12.0 * n, g_ms)
-    copy = T.time(lambda: ctx.stream_op(edpp.STREAM_COPY, src, None, dst, n), reps=10)
+    copy = T.time(lambda: ctx.stream_op(edpp.STREAM_COPY, src, None, dst, n), reps=20)
     copy_gbs = gbs(8.0 * n, copy["hampel_ms"])
 
     line = {
@@ -368,21 +423,31 @@ def suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs):
     ctx.gen_i32(b, 0x11)
     st = {}
     for name, op, nb in [("copy", 0, 8), ("scale", 1, 8), ("add", 2, 12), ("triad", 3, 12)]:
-        t = T.time(lambda: ctx.stream_op(op, src, b, dst, n), reps=10)
+        t = T.time(lambda: ctx.stream_op(op, src, b, dst, n), reps=20)
         st[name] = round(gbs(nb * n, t["hampel_ms"]), 2)
     res["stream_gbs"] = st
     print("stream", st, file=sys.stderr, flush=True)
-    # scans: C1 (2^24, values in [0,15]) and the metric size 2^28
+    # scans: 2^24 (values in [0,15]) and 2^28; warm (back to back) and cold
+    # (the 256 MiB Infinity Cache evicted by a 1 GiB write before each run):
+    # 2^24 in + out is 128 MiB, so only its cold runs price HBM
     sc = {}
+    scratch = torch.empty(1 << 28, dtype=i32, device="cuda")
+    flush = lambda: scratch.fill_(0x5A5A)
     for lg in (24, 28):
         m = 1 << lg
         x = b[:m]
         ctx.gen_i32(x, 2024, 15)
         for name, scheme in [("lookback", 1), ("rts", 2), ("ssa", 3)]:
-            t = T.time(lambda: ctx.scan(x, dst, m, scheme), reps=10)
+            t = T.time(lambda: ctx.scan(x, dst, m, scheme), reps=20)
+            tc = T.time(lambda: ctx.scan(x, dst, m, scheme), reps=20, flush=flush)
             g = gbs(8.0 * m, t["hampel_ms"])
+            gc = gbs(8.0 * m, tc["hampel_ms"])
             sc[f"2^{lg}_{name}"] = {"gbs": round(g, 2), "ms": round(t["hampel_ms"], 4),
-                                    "frac_peak": round(g / HBM_PEAK_GBS, 4), "frac_copy": round(g / copy_gbs, 4)}
+                                    "median_ms": round(t["median_ms"], 4), "min_ms": round(t["min_ms"], 4),
+                                    "frac_peak": round(g / HBM_PEAK_GBS, 4), "frac_copy": round(g / copy_gbs, 4),
+                                    "cold_gbs": round(gc, 2), "cold_ms": round(tc["hampel_ms"], 4),
+                                    "cold_frac_copy": round(gc / copy_gbs, 4)}
+    del scratch
     res["scan"] = sc
     print("scan", sc, file=sys.stderr, flush=True)
     # gather / scatter over pass counts and index kinds
@@ -390,8 +455,8 @@ def suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs):
     for kind, kname in [(edpp.PERM_RANDOM, "random"), (edpp.PERM_BLOCK_LOCAL, "block4096")]:
         ctx.gen_perm(idx, 0x5EED, kind)
         for p in (1, 2, 4, 8, 16):
-            tg = T.time(lambda: ctx.gather(src, dst, idx, n, p), reps=5)
-            ts = T.time(lambda: ctx.scatter(src, dst, idx, n, p), reps=5)
+            tg = T.time(lambda: ctx.gather(src, dst, idx, n, p), reps=10)
+            ts = T.time(lambda: ctx.scatter(src, dst, idx, n, p), reps=10)
             gs[f"{kname}_p{p}"] = {"gather_gbs": round(gbs(12.0 * n, tg["hampel_ms"]), 2),
                                    "scatter_gbs": round(gbs(12.0 * n, ts["hampel_ms"]), 2)}
     ctx.gen_perm(idx, 0x5EED, edpp.PERM_RANDOM)
@@ -405,7 +470,7 @@ def suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs):
     for B in (2, 16, 256, 4096):
         for dist, dname in [(0, "uniform"), (1, "zipf")]:
             ctx.gen_split_keys(src, b, B, dist, 0x5917 + B)
-            t = T.time(lambda: ctx.split(src, ko, B, b, vo, start, n, edpp.SPLIT_AUTO), reps=5)
+            t = T.time(lambda: ctx.split(src, ko, B, b, vo, start, n, edpp.SPLIT_AUTO), reps=20)
             g = gbs(16.0 * n, t["hampel_ms"])
             sp[f"B{B}_{dname}"] = {"gbs": round(g, 2), "ms": round(t["hampel_ms"], 3),
                                    "frac_copy": round(g / copy_gbs, 4)}
@@ -444,7 +509,7 @@ def suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs):
     for name, fn in [("pairs_u32", lambda: ctx.sort(src, ko, b, vo, n, 0, 32)),
                      ("pairs_i32", lambda: ctx.sort(src, ko, b, vo, n, 0, 32, True)),
                      ("keys_u32", lambda: ctx.sort(src, ko, None, None, n, 0, 32))]:
-        t = T.time(fn, reps=5)
+        t = T.time(fn, reps=10)
         per = 16.0 if name.startswith("pairs") else 8.0  # one read and one write of each pair
         srt[name] = {"ms": round(t["hampel_ms"], 3), "gkeys_s": round(n / (t["hampel_ms"] * 1e-3) / 1e9, 2),
                      "gbs_one_pass_bytes": round(gbs(per * n, t["hampel_ms"]), 1)}
