@@ -54,7 +54,7 @@ def parse():
     p.add_argument("--out", default=None, help="also write the JSON line to this file")
     p.add_argument("--profile", default=None,
                    choices=["stream", "gather", "scatter", "gather_local", "scatter_local", "scan", "split", "sort",
-                            "join", "batched"],
+                            "join", "batched", "mlp"],
                    help="only run one workload at full size (for rocprofv3), plus one STREAM copy")
     return p.parse_args()
 
@@ -273,6 +273,27 @@ def profile_only(args, ctx, edpp, torch):
         cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
         for _ in range(reps):
             ctx.join_count(a[:m], b[:m], cnt, m, m)
+    elif args.profile == "mlp":
+        # the cost model's HIP kernels at the learned run's shapes: training on
+        # 2000 rows x 1500 epochs, inference on 500 rows, and the per-call
+        # selection (features + forward) of AUTO calls with the memo off
+        rng = np.random.default_rng(5)
+        x = torch.from_numpy(rng.normal(size=(2000, 12)).astype(np.float32)).cuda()
+        t = torch.from_numpy(rng.dirichlet(np.ones(15), size=2000).astype(np.float32)).cuda()
+        msk = torch.ones(2000, 15, dtype=torch.float32, device="cuda")
+        mlp = edpp.MLP(ctx, 7)
+        for _ in range(reps):
+            mlp.train(x, t, msk, 1500, 3e-3)
+        choice = torch.empty(500, dtype=torch.int32, device="cuda")
+        for _ in range(100):
+            mlp.forward(x[:500], msk[:500], None, choice)
+        ctx.set_model(mlp)
+        mlp.memo(False)
+        m = 1 << 20
+        for _ in range(100):
+            ctx.scan(a[:m], b[:m], m, edpp.SCAN_AUTO)
+        ctx.set_model(None)
+        mlp.close()
     elif args.profile == "batched":
         del a, b
         batched(ctx, edpp, torch)

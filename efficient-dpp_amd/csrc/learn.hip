@@ -30,11 +30,12 @@ struct edpp_mlp {
     uint64_t memo_clock = 0;
     ChoiceMemo memo[kMemoSlots];
     float* params = nullptr;   // device, P floats
-    float* grad = nullptr;     // device, P floats
     float* partial = nullptr;  // device, kGradGroups * (P + 1) floats
     float* adam_m = nullptr;
     float* adam_v = nullptr;
     float* loss = nullptr;     // device, 1 float
+    float* losses = nullptr;   // device, the per-epoch losses of one training call
+    int32_t losses_cap = 0;
     int32_t* pinned_choice = nullptr;  // host-pinned, written by the selection kernel
     int64_t adam_step = 0;
 };
@@ -46,7 +47,7 @@ namespace {
 constexpr int F = EDPP_NUM_FEATURES, H = EDPP_HIDDEN, K = EDPP_MAX_CANDIDATES;
 constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + H * F, OFF_W2 = OFF_B1 + H, OFF_B2 = OFF_W2 + H * H,
               OFF_W3 = OFF_B2 + H, OFF_B3 = OFF_W3 + K * H, NPARAM = OFF_B3 + K;
-constexpr int kGradGroups = 64;
+constexpr int kGradGroups = 256;  // most workgroups (and partials) of one gradient
 
 struct Cand {
     int family;
@@ -202,97 +203,217 @@ __global__ __launch_bounds__(64) void k_mlp_forward(const float* __restrict__ P,
     }
 }
 
-// Gradient of the mean cross-entropy: workgroup g accumulates rows
-// g, g+G, ... in registers (lane j owns row j of W1 and W2, column j of W3)
-// and writes its partial sums; k_mlp_reduce adds the partials in a fixed
-// order, so the result is deterministic.
-__global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ P, const float* __restrict__ X,
-                                                 const float* __restrict__ T, const float* __restrict__ M,
-                                                 int32_t rows, float* __restrict__ partial) {
-    __shared__ float x[F], mask[K], tgt[K], h1[H], h2[H], z[K], p[K], dz[K], dh2[H];
-    __shared__ int ch;
-    const int j = threadIdx.x;
-    float gW1[F], gW2[H], gW3[K];
-    float gb1 = 0.f, gb2 = 0.f, gb3 = 0.f, loss = 0.f;
+// Gradient of the mean cross-entropy, one workgroup of kGradBlock lanes per
+// group of chunks of kGradRows consecutive rows (chunks g, g+G, ... for
+// workgroup g of G).  The weights are staged once per workgroup in LDS,
+// transposed with odd row strides (W1T[k][j], W2T[k][j], W3T[k][c]), so that
+// every layer reads them conflict-free both forward (lanes over the output
+// unit) and backward (lanes over the input unit).  A chunk goes through the
+// network as small matrices: wave w takes rows w, w+4, ... of each layer with
+// one lane per unit; the logits take one lane per (row, candidate).  Each lane
+// keeps its share of the weight gradients in registers over all the
+// workgroup's chunks and writes them once as the workgroup's partial sums;
+// k_mlp_reduce and k_mlp_reduce_adam add the partials in a fixed order, so
+// the result is deterministic.
+constexpr int kGradBlock = 256;
+constexpr int kGradRows = 16;
+constexpr int kGradWaves = kGradBlock / kWave;  // 4
+constexpr int SF = F + 1, SH = H + 1, SK = 17;   // LDS row strides (odd: no bank conflicts)
+static_assert(H == kWave && K < 16 && F % kGradWaves == 0 && H % kGradWaves == 0, "lane mapping of k_mlp_grad");
+static_assert(kGradRows * 16 == kGradBlock && kGradRows % kGradWaves == 0, "lane mapping of k_mlp_grad");
+
+__global__ __launch_bounds__(kGradBlock) void k_mlp_grad(const float* __restrict__ P, const float* __restrict__ X,
+                                                         const float* __restrict__ T, const float* __restrict__ M,
+                                                         int32_t rows, float* __restrict__ partial) {
+    constexpr int R = kGradRows, RW = R / kGradWaves;
+    __shared__ float sW1T[F * SH], sW2T[H * SH], sW3T[H * SK], sb1[H], sb2[H], sb3[16];
+    __shared__ float sX[R * SF], sM[R * 16], sT[R * 16], sH1[R * SH], sH2[R * SH], sdH2[R * SH], sdH1[R * SH],
+        sdZ[R * SK], sLoss[kGradBlock / 16];
+    const int t = threadIdx.x, w = t / kWave, l = t % kWave;
+    for (int i = t; i < H * F; i += kGradBlock) sW1T[(i % F) * SH + i / F] = P[OFF_W1 + i];
+    for (int i = t; i < H * H; i += kGradBlock) sW2T[(i % H) * SH + i / H] = P[OFF_W2 + i];
+    for (int i = t; i < K * H; i += kGradBlock) sW3T[(i % H) * SK + i / H] = P[OFF_W3 + i];
+    if (t < H) {
+        sb1[t] = P[OFF_B1 + t];
+        sb2[t] = P[OFF_B2 + t];
+    }
+    if (t < K) sb3[t] = P[OFF_B3 + t];
+    constexpr int QK = (K + kGradWaves - 1) / kGradWaves;
+    float gW2[H / kGradWaves], gW1[F / kGradWaves], gW3[QK], gb = 0.f, loss = 0.f;
     #pragma unroll
-    for (int k = 0; k < F; ++k) gW1[k] = 0.f;
-    for (int k = 0; k < H; ++k) gW2[k] = 0.f;
+    for (int q = 0; q < H / kGradWaves; ++q) gW2[q] = 0.f;
     #pragma unroll
-    for (int c = 0; c < K; ++c) gW3[c] = 0.f;
+    for (int q = 0; q < QK; ++q) gW3[q] = 0.f;
+    #pragma unroll
+    for (int q = 0; q < F / kGradWaves; ++q) gW1[q] = 0.f;
     const float inv = 1.0f / (float)rows;
-    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
-        if (j < F) x[j] = X[(int64_t)r * F + j];
-        if (j < K) {
-            mask[j] = M[(int64_t)r * K + j];
-            tgt[j] = T[(int64_t)r * K + j];
+    const int64_t chunks = ((int64_t)rows + R - 1) / R;
+    for (int64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
+        const int64_t r0 = ch * R;
+        __syncthreads();  // weights staged; the previous chunk's reads are done
+        for (int i = t; i < R * F; i += kGradBlock) {
+            const int r = i / F, k = i % F;
+            sX[r * SF + k] = r0 + r < rows ? X[(r0 + r) * F + k] : 0.f;
+        }
+        for (int i = t; i < R * K; i += kGradBlock) {
+            const int r = i / K, c = i % K;
+            const bool ok = r0 + r < rows;
+            sM[r * 16 + c] = ok ? M[(r0 + r) * K + c] : 0.f;  // rows past the end: no candidate, no gradient
+            sT[r * 16 + c] = ok ? T[(r0 + r) * K + c] : 0.f;
         }
         __syncthreads();
-        mlp_row_forward(P, x, mask, h1, h2, z, p, &ch);
-        if (j < K) dz[j] = mask[j] > 0.0f ? (p[j] - tgt[j]) * inv : 0.0f;
-        if (j == 0) {
-            float l = 0.f;
-            for (int c = 0; c < K; ++c)
-                if (mask[c] > 0.0f && tgt[c] > 0.0f) l -= tgt[c] * logf(fmaxf(p[c], 1e-30f));
-            loss += l * inv;
+        #pragma unroll 1
+        for (int q = 0; q < RW; ++q) {  // hidden layer 1: lane l = unit j
+            const int r = w + kGradWaves * q;
+            float a = sb1[l];
+            #pragma unroll
+            for (int k = 0; k < F; ++k) a += sW1T[k * SH + l] * sX[r * SF + k];
+            sH1[r * SH + l] = fmaxf(a, 0.0f);
         }
         __syncthreads();
-        // output layer: lane j owns column j of W3 (all K rows) and b3[j]
-        float d2 = 0.f;
-        #pragma unroll
-        for (int c = 0; c < K; ++c) {
-            gW3[c] += dz[c] * h2[j];
-            d2 += P[OFF_W3 + c * H + j] * dz[c];
+        #pragma unroll 1
+        for (int q = 0; q < RW; ++q) {  // hidden layer 2
+            const int r = w + kGradWaves * q;
+            float a = sb2[l];
+            #pragma unroll 8
+            for (int k = 0; k < H; ++k) a += sW2T[k * SH + l] * sH1[r * SH + k];
+            sH2[r * SH + l] = fmaxf(a, 0.0f);
         }
-        if (j < K) gb3 += dz[j];
-        d2 = h2[j] > 0.0f ? d2 : 0.0f;
-        dh2[j] = d2;
         __syncthreads();
-        // hidden layer 2: lane j owns row j of W2
-        for (int k = 0; k < H; ++k) gW2[k] += d2 * h1[k];
-        gb2 += d2;
-        float d1 = 0.f;
-        for (int i = 0; i < H; ++i) d1 += P[OFF_W2 + i * H + j] * dh2[i];
-        d1 = h1[j] > 0.0f ? d1 : 0.0f;
-        #pragma unroll
-        for (int k = 0; k < F; ++k) gW1[k] += d1 * x[k];
-        gb1 += d1;
+        {  // logits, masked softmax and its gradient: lane t = (row t/16, candidate t%16)
+            const int r = t / 16, c = t % 16;
+            const bool valid = c < K && sM[r * 16 + c] > 0.0f;
+            float z = 0.f;
+            if (c < K) {
+                z = sb3[c];
+                #pragma unroll 8
+                for (int k = 0; k < H; ++k) z += sW3T[k * SK + c] * sH2[r * SH + k];
+            }
+            float mx = valid ? z : -INFINITY;
+            #pragma unroll
+            for (int o = 8; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+            const float e = valid ? expf(z - mx) : 0.f;
+            float s = e;
+            #pragma unroll
+            for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o);
+            const float p = s > 0.f ? e / s : 0.f;
+            const float tg = c < K ? sT[r * 16 + c] : 0.f;
+            sdZ[r * SK + c] = valid ? (p - tg) * inv : 0.f;
+            float lt = valid && tg > 0.f ? -tg * logf(fmaxf(p, 1e-30f)) : 0.f;
+            #pragma unroll
+            for (int o = 8; o > 0; o >>= 1) lt += __shfl_xor(lt, o);
+            if (c == 0) loss += lt * inv;
+        }
         __syncthreads();
+        #pragma unroll 1
+        for (int q = 0; q < RW; ++q) {  // back through W3 and the ReLU of layer 2
+            const int r = w + kGradWaves * q;
+            float d = 0.f;
+            #pragma unroll
+            for (int c = 0; c < K; ++c) d += sdZ[r * SK + c] * sW3T[l * SK + c];
+            sdH2[r * SH + l] = sH2[r * SH + l] > 0.0f ? d : 0.0f;
+        }
+        __syncthreads();
+        #pragma unroll 1
+        for (int q = 0; q < RW; ++q) {  // back through W2 and the ReLU of layer 1: lane l = input unit k
+            const int r = w + kGradWaves * q;
+            float d = 0.f;
+            #pragma unroll 8
+            for (int j = 0; j < H; ++j) d += sdH2[r * SH + j] * sW2T[l * SH + j];
+            sdH1[r * SH + l] = sH1[r * SH + l] > 0.0f ? d : 0.0f;
+        }
+        __syncthreads();
+        // weight and bias gradients of this chunk, one row at a time in order:
+        // W2[j][k] (j = w + 4q, k = l), W1[j][k] (j = l, k = w + 4q),
+        // W3[c][j] (c = w + 4q, j = l); biases: wave 0 b1, wave 1 b2, wave 2 b3
+        #pragma unroll 1
+        for (int r = 0; r < R; ++r) {
+            const float h1 = sH1[r * SH + l], h2 = sH2[r * SH + l], d1 = sdH1[r * SH + l];
+            #pragma unroll
+            for (int q = 0; q < H / kGradWaves; ++q) gW2[q] += sdH2[r * SH + w + kGradWaves * q] * h1;
+            #pragma unroll
+            for (int q = 0; q < F / kGradWaves; ++q) gW1[q] += d1 * sX[r * SF + w + kGradWaves * q];
+            #pragma unroll
+            for (int q = 0; q < (K + kGradWaves - 1) / kGradWaves; ++q)
+                if (w + kGradWaves * q < K) gW3[q] += sdZ[r * SK + w + kGradWaves * q] * h2;
+            gb += w == 0 ? d1 : (w == 1 ? sdH2[r * SH + l] : (w == 2 && l < K ? sdZ[r * SK + l] : 0.f));
+        }
     }
     float* out = partial + (int64_t)blockIdx.x * (NPARAM + 1);
     #pragma unroll
-    for (int k = 0; k < F; ++k) out[OFF_W1 + j * F + k] = gW1[k];
-    out[OFF_B1 + j] = gb1;
-    for (int k = 0; k < H; ++k) out[OFF_W2 + j * H + k] = gW2[k];
-    out[OFF_B2 + j] = gb2;
+    for (int q = 0; q < H / kGradWaves; ++q) out[OFF_W2 + (w + kGradWaves * q) * H + l] = gW2[q];
     #pragma unroll
-    for (int c = 0; c < K; ++c) out[OFF_W3 + c * H + j] = gW3[c];
-    if (j < K) out[OFF_B3 + j] = gb3;
-    if (j == 0) out[NPARAM] = loss;
-}
-
-__global__ void k_mlp_reduce(const float* __restrict__ partial, int groups, float* __restrict__ grad, float* loss) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i <= NPARAM; i += gridDim.x * blockDim.x) {
+    for (int q = 0; q < F / kGradWaves; ++q) out[OFF_W1 + l * F + w + kGradWaves * q] = gW1[q];
+    #pragma unroll
+    for (int q = 0; q < QK; ++q)
+        if (w + kGradWaves * q < K) out[OFF_W3 + (w + kGradWaves * q) * H + l] = gW3[q];
+    if (w == 0) out[OFF_B1 + l] = gb;
+    if (w == 1) out[OFF_B2 + l] = gb;
+    if (w == 2 && l < K) out[OFF_B3 + l] = gb;
+    // the loss: lanes t = 16*row hold it, added in row order
+    if (t % 16 == 0) sLoss[t / 16] = loss;
+    __syncthreads();
+    if (t == 0) {
         float s = 0.f;
-        for (int g = 0; g < groups; ++g) s += partial[(int64_t)g * (NPARAM + 1) + i];
+        for (int q = 0; q < kGradBlock / 16; ++q) s += sLoss[q];
+        out[NPARAM] = s;
+    }
+}
+
+// Sum of entry i over the first `groups` partials.  A block of 256 lanes
+// takes 64 entries; the four waves each add a quarter of the partials
+// (partials q, q+4, q+8, ... in wave q, as two interleaved chains) and the
+// quarters are combined in a fixed order, so every call returns the same
+// float.  Valid in wave 0; every lane of the block must call it.
+constexpr int kReduceEntries = 64;
+__device__ __forceinline__ float sum_partials(const float* __restrict__ partial, int groups, int i, float* lds) {
+    const int q = threadIdx.x / kWave;
+    float a0 = 0.f, a1 = 0.f;
+    if (i <= NPARAM) {
+        int g = q;
+        for (; g + 4 < groups; g += 8) {
+            a0 += partial[(int64_t)g * (NPARAM + 1) + i];
+            a1 += partial[(int64_t)(g + 4) * (NPARAM + 1) + i];
+        }
+        if (g < groups) a0 += partial[(int64_t)g * (NPARAM + 1) + i];
+    }
+    lds[threadIdx.x] = a0 + a1;
+    __syncthreads();
+    const int e = threadIdx.x % kWave;
+    return (lds[e] + lds[kWave + e]) + (lds[2 * kWave + e] + lds[3 * kWave + e]);
+}
+
+__global__ __launch_bounds__(256) void k_mlp_reduce(const float* __restrict__ partial, int groups,
+                                                    float* __restrict__ grad, float* loss) {
+    __shared__ float lds[256];
+    const int i = blockIdx.x * kReduceEntries + (int)(threadIdx.x % kWave);
+    const float s = sum_partials(partial, groups, i, lds);
+    if (threadIdx.x < kWave) {
         if (i < NPARAM) grad[i] = s;
-        else if (loss) *loss = s;
+        else if (i == NPARAM && loss) *loss = s;
     }
 }
 
-__global__ void k_adam(float* __restrict__ P, const float* __restrict__ g, float* __restrict__ m,
-                       float* __restrict__ v, float lr, float c1, float c2) {
+// The reduction fused with the Adam step of training: parameter i is updated
+// from its summed gradient; the loss goes to loss_at (the epoch's slot).
+__global__ __launch_bounds__(256) void k_mlp_reduce_adam(const float* __restrict__ partial, int groups,
+                                                         float* __restrict__ P, float* __restrict__ m,
+                                                         float* __restrict__ v, float lr, float c1, float c2,
+                                                         float* loss_at) {
+    __shared__ float lds[256];
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < NPARAM; i += gridDim.x * blockDim.x) {
-        float mi = b1 * m[i] + (1.f - b1) * g[i];
-        float vi = b2 * v[i] + (1.f - b2) * g[i] * g[i];
-        m[i] = mi;
-        v[i] = vi;
-        P[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    const int i = blockIdx.x * kReduceEntries + (int)(threadIdx.x % kWave);
+    const float g = sum_partials(partial, groups, i, lds);
+    if (threadIdx.x >= kWave || i > NPARAM) return;
+    if (i == NPARAM) {
+        *loss_at = g;
+        return;
     }
-}
-
-__global__ void k_copy_f32(float* dst, const float* src, int n) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i];
+    const float mi = b1 * m[i] + (1.f - b1) * g;
+    const float vi = b2 * v[i] + (1.f - b2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    P[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
 // features + one forward row + masked argmax, result to host-pinned memory
@@ -577,7 +698,7 @@ int edpp_mlp_create(edpp_ctx* c, uint64_t seed, edpp_mlp** out) {
     edpp_mlp* m = new edpp_mlp();
     m->ctx = c;
     const size_t pb = sizeof(float) * NPARAM;
-    if (hipMalloc(&m->params, pb) != hipSuccess || hipMalloc(&m->grad, pb) != hipSuccess ||
+    if (hipMalloc(&m->params, pb) != hipSuccess ||
         hipMalloc(&m->adam_m, pb) != hipSuccess || hipMalloc(&m->adam_v, pb) != hipSuccess ||
         hipMalloc(&m->loss, sizeof(float)) != hipSuccess ||
         hipMalloc(&m->partial, sizeof(float) * (size_t)kGradGroups * (NPARAM + 1)) != hipSuccess ||
@@ -609,10 +730,10 @@ int edpp_mlp_destroy(edpp_mlp* m) {
     if (m->ctx && m->ctx->model == m) m->ctx->model = nullptr;
     if (m->ctx) hipStreamSynchronize(m->ctx->stream);
     hipFree(m->params);
-    hipFree(m->grad);
     hipFree(m->adam_m);
     hipFree(m->adam_v);
     hipFree(m->loss);
+    hipFree(m->losses);
     hipFree(m->partial);
     if (m->pinned_choice) hipHostFree(m->pinned_choice);
     delete m;
@@ -651,13 +772,21 @@ int edpp_mlp_forward(edpp_mlp* m, const float* x, const float* mask, int32_t row
     return EDPP_OK;
 }
 
+// workgroups of k_mlp_grad for `rows` rows (one partial each)
+static int grad_groups(int32_t rows) {
+    return (int)std::min<int64_t>(((int64_t)rows + kGradRows - 1) / kGradRows, kGradGroups);
+}
+
+constexpr int kReduceBlocks = (NPARAM + 1 + kReduceEntries - 1) / kReduceEntries;
+
 static int mlp_grad(edpp_mlp* m, const float* x, const float* t, const float* mask, int32_t rows, float* loss,
                     float* grad) {
     edpp_ctx* c = m->ctx;
-    hipLaunchKernelGGL(k_mlp_grad, dim3(kGradGroups), dim3(64), 0, c->stream, (const float*)m->params, x, t, mask,
+    const int groups = grad_groups(rows);
+    hipLaunchKernelGGL(k_mlp_grad, dim3(groups), dim3(kGradBlock), 0, c->stream, (const float*)m->params, x, t, mask,
                        rows, m->partial);
-    hipLaunchKernelGGL(k_mlp_reduce, dim3(24), dim3(256), 0, c->stream, (const float*)m->partial, kGradGroups, grad,
-                       loss);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3(kReduceBlocks), dim3(256), 0, c->stream, (const float*)m->partial, groups,
+                       grad, loss);
     EDPP_HIP(c, hipGetLastError());
     return EDPP_OK;
 }
@@ -673,16 +802,31 @@ int edpp_mlp_train(edpp_mlp* m, const float* x, const float* t, const float* mas
     if (!m || rows <= 0 || epochs < 0 || !x || !t || !mask) return EDPP_EINVAL;
     edpp_ctx* c = m->ctx;
     ++m->version;
+    if (epochs > m->losses_cap) {  // per-epoch losses stay on the device until the end
+        EDPP_HIP(c, hipStreamSynchronize(c->stream));
+        hipFree(m->losses);
+        m->losses = nullptr;
+        m->losses_cap = 0;
+        if (hipMalloc(&m->losses, sizeof(float) * (size_t)epochs) != hipSuccess) {
+            m->losses = nullptr;
+            return fail(c, EDPP_ENOMEM, "mlp loss buffer of %d epochs", epochs);
+        }
+        m->losses_cap = epochs;
+    }
+    const int groups = grad_groups(rows);
     for (int e = 0; e < epochs; ++e) {
-        int rc = mlp_grad(m, x, t, mask, rows, m->loss, m->grad);
-        if (rc) return rc;
+        hipLaunchKernelGGL(k_mlp_grad, dim3(groups), dim3(kGradBlock), 0, c->stream, (const float*)m->params, x, t,
+                           mask, rows, m->partial);
         ++m->adam_step;
         const float c1 = 1.0f - powf(0.9f, (float)m->adam_step), c2 = 1.0f - powf(0.999f, (float)m->adam_step);
-        hipLaunchKernelGGL(k_adam, dim3(24), dim3(256), 0, c->stream, m->params, (const float*)m->grad, m->adam_m,
-                           m->adam_v, lr, c1, c2);
-        if (loss_host) EDPP_HIP(c, hipMemcpyAsync(loss_host + e, m->loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(kReduceBlocks), dim3(256), 0, c->stream,
+                           (const float*)m->partial, groups, m->params, m->adam_m, m->adam_v, lr, c1, c2,
+                           m->losses + e);
     }
     EDPP_HIP(c, hipGetLastError());
+    if (loss_host && epochs > 0)
+        EDPP_HIP(c, hipMemcpyAsync(loss_host, m->losses, sizeof(float) * (size_t)epochs, hipMemcpyDeviceToHost,
+                                   c->stream));
     EDPP_HIP(c, hipStreamSynchronize(c->stream));
     return EDPP_OK;
 }
