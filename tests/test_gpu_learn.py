@@ -27,7 +27,9 @@ def test_mlp_forward_and_gradient_match_reference(torch_gpu, ctx, edpp):
     torch = torch_gpu
     mlp = edpp.MLP(ctx, seed=3)
     p = mlp.get_params().astype(np.float64)
-    for rows in (1, 37, 300):
+    # 16 and 17 rows: one whole chunk of the gradient kernel and one row past it;
+    # 5000 rows: more chunks than workgroups, so workgroups take several
+    for rows in (1, 16, 17, 37, 300, 5000):
         x, t, mask = _rand_problem(rows, rows)
         dx, dt, dm = (torch.from_numpy(a).cuda() for a in (x, t, mask))
         probs = torch.empty((rows, 15), dtype=torch.float32, device="cuda")
@@ -43,6 +45,26 @@ def test_mlp_forward_and_gradient_match_reference(torch_gpu, ctx, edpp):
         g = mlp_ref.grad(p, x, t, mask)
         assert np.allclose(grad.cpu().numpy(), g, rtol=1e-3, atol=1e-5 * max(1.0, np.abs(g).max()))
     mlp.close()
+
+
+def test_mlp_gradient_and_training_are_deterministic(torch_gpu, ctx, edpp):
+    # partial gradients are summed in a fixed order: repeated calls agree bit for bit
+    torch = torch_gpu
+    x, t, mask = _rand_problem(5000, 11)
+    dx, dt, dm = (torch.from_numpy(a).cuda() for a in (x, t, mask))
+    runs = []
+    for _ in range(2):
+        mlp = edpp.MLP(ctx, seed=5)
+        loss = torch.empty(1, dtype=torch.float32, device="cuda")
+        grad = torch.empty(mlp.nparams, dtype=torch.float32, device="cuda")
+        mlp.loss_grad(dx, dt, dm, loss, grad)
+        losses = mlp.train(dx, dt, dm, epochs=50, lr=3e-3)
+        runs.append((grad.cpu().numpy(), float(loss.item()), losses, mlp.get_params()))
+        mlp.close()
+    (g0, l0, h0, p0), (g1, l1, h1, p1) = runs
+    assert np.array_equal(g0, g1) and l0 == l1
+    assert np.array_equal(h0, h1) and np.array_equal(p0, p1)
+    assert h0[0] == pytest.approx(l0, rel=1e-6)  # the first epoch's loss is the loss at the start
 
 
 def test_mlp_training_learns_a_rule(torch_gpu, ctx, edpp):
