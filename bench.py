@@ -145,47 +145,47 @@ def cpu_baseline(args, torch_unused=None):
                   f"Hampel mean of {len(times)} runs ({ms:.1f} ms); host {cpu}, nproc={os.cpu_count()}, "
                   f"sockets={len(sockets) or 'unknown'}, OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')}, "
                   f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND', 'unset')}",
-        "primitives": cpu_primitives(orc, ref, b, 1 << 25),
+        "primitives": cpu_primitives(orc, ref, b, 1 << 27, src, idx),
     }
 
 
-def cpu_primitives(orc, ref, b, n):
-    """The other host primitives of the CPU baseline on n int32: the scans
-    (scan-scan-add and reduce-then-scan, openmp/test_scan_CPU.cpp:52-163,
-    restated: that file needs oneTBB), copy and scale
-    (openmp/test_bandwidth_CPU.cpp:39-83, the reference's own when built),
-    gather and scatter through a random permutation.  Run 0 is first touch
-    and discarded, then 5 timed runs; GB/s from the Hampel mean."""
+def cpu_primitives(orc, ref, b, n, gsrc, gidx):
+    """The other host primitives of the CPU baseline: the scans (scan-scan-add
+    and reduce-then-scan, openmp/test_scan_CPU.cpp:52-163, restated: that file
+    needs oneTBB), copy and scale (openmp/test_bandwidth_CPU.cpp:39-83, the
+    reference's own when built) on n int32, and gather and scatter apart on
+    the gather/scatter sample (gsrc, gidx).  Run 0 is first touch and
+    discarded, then 5 timed runs; GB/s from the Hampel mean."""
     src = orc.gen_i32(n, 0x5CA7) & 15
     out = np.empty(n, np.int32)
-    idx = orc.gen_perm(n, 0xC0FFEE, 0)
+    m = gsrc.size
+    gout = np.empty(m, np.int32)
     A = orc._a
     if ref is not None:
         copy = lambda: ref.edpp_ref_copy(A(src), A(out), n)
         scale = lambda: ref.edpp_ref_scale(A(src), A(out), n)
-        gather = lambda: ref.edpp_ref_gather(A(src), A(out), A(idx), n)
-        scatter = lambda: ref.edpp_ref_scatter(A(src), A(out), A(idx), n)
+        gather = lambda: ref.edpp_ref_gather(A(gsrc), A(gout), A(gidx), m)
+        scatter = lambda: ref.edpp_ref_scatter(A(gsrc), A(gout), A(gidx), m)
     else:
         copy = lambda: b.baseline_copy(A(src), A(out), n)
         scale = lambda: b.baseline_scale(A(src), A(out), n)
-        gather = lambda: b.baseline_gather(A(src), A(out), A(idx), n)
-        scatter = lambda: b.baseline_scatter(A(src), A(out), A(idx), n)
+        gather = lambda: b.baseline_gather(A(gsrc), A(gout), A(gidx), m)
+        scatter = lambda: b.baseline_scatter(A(gsrc), A(gout), A(gidx), m)
     res = {}
-    for name, fn, nb, kind in [("scan_ssa", lambda: b.baseline_scan_ssa(A(src), A(out), n), 8, "port"),
-                               ("scan_rts", lambda: b.baseline_scan_rts(A(src), A(out), n), 8, "port"),
-                               ("copy", copy, 8, "reference" if ref is not None else "port"),
-                               ("scale", scale, 8, "reference" if ref is not None else "port"),
-                               ("gather", gather, 12, "reference" if ref is not None else "port"),
-                               ("scatter", scatter, 12, "reference" if ref is not None else "port")]:
+    for name, fn, nb, cnt, kind in [("scan_ssa", lambda: b.baseline_scan_ssa(A(src), A(out), n), 8, n, "port"),
+                                    ("scan_rts", lambda: b.baseline_scan_rts(A(src), A(out), n), 8, n, "port"),
+                                    ("copy", copy, 8, n, "reference" if ref is not None else "port"),
+                                    ("scale", scale, 8, n, "reference" if ref is not None else "port"),
+                                    ("gather", gather, 12, m, "reference" if ref is not None else "port"),
+                                    ("scatter", scatter, 12, m, "reference" if ref is not None else "port")]:
         fn()
         ts = [fn() for _ in range(5)]
-        res[name] = {"gbs": round(gbs(nb * n, hampel(ts)), 3), "median_ms": round(float(np.median(ts)), 3),
-                     "kind": kind}
+        res[name] = {"n": cnt, "gbs": round(gbs(nb * cnt, hampel(ts)), 3),
+                     "median_ms": round(float(np.median(ts)), 3), "kind": kind}
         if name.startswith("scan"):  # exclusive prefix, spot-checked at both ends
             head = np.concatenate(([0], np.cumsum(src[:4095], dtype=np.int64))).astype(np.int32)
             tail = int(np.sum(src[:-1], dtype=np.int64) & 0xFFFFFFFF)
             res[name]["checked"] = bool(np.array_equal(out[:4096], head)) and int(out[-1]) & 0xFFFFFFFF == tail
-    res["n"] = n
     if os.path.exists("/usr/include/tbb/task_scheduler_init.h"):
         res["tbb_scan"] = "not built: the TBB headers are present but this baseline does not link TBB"
     elif os.path.isdir("/usr/include/tbb") or os.path.isdir("/usr/include/oneapi/tbb"):
