@@ -37,6 +37,7 @@ _SIGS = {
     "edpp_ctx_create": [c_int, c_vp, ctypes.POINTER(c_vp)],
     "edpp_ctx_destroy": [c_vp],
     "edpp_ctx_sync": [c_vp],
+    "edpp_ctx_set_watchdog": [c_vp, ctypes.c_uint32],
     "edpp_ctx_stream": [c_vp],
     "edpp_ctx_device_info": [c_vp, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_i64),
                              ctypes.POINTER(c_i64)],
@@ -260,6 +261,11 @@ class Context:
 
     def sync(self):
         self._check(self._L.edpp_ctx_sync(self.h))
+
+    def set_watchdog(self, polls=1 << 22):
+        """Polls of a device-side wait before it gives up and the next sync
+        raises EdppError(ETIMEOUT); the default restores the library's."""
+        self._check(self._L.edpp_ctx_set_watchdog(self.h, int(polls)))
 
     @property
     def stream(self):

@@ -297,6 +297,12 @@ int edpp_ctx_sync(edpp_ctx* c) {
     return check_device_error(c);
 }
 
+int edpp_ctx_set_watchdog(edpp_ctx* c, uint32_t polls) {
+    EDPP_CHECK_CTX(c);
+    c->spin_limit = polls;
+    return EDPP_OK;
+}
+
 void* edpp_ctx_stream(edpp_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
 int edpp_ctx_device_info(edpp_ctx* c, int* cus, int* xcds, int64_t* l2, int64_t* mall) {

@@ -92,8 +92,11 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe(const uint32_t* __res
                                                            unsigned long long* d_count) {
     extern __shared__ uint64_t table[];  // kJoinSlots
     const int b = blockIdx.x;
-    const int64_t r0 = rs[b], r1 = b + 1 < B ? rs[b + 1] : rn;
-    const int64_t s0 = ss[b], s1 = b + 1 < B ? ss[b + 1] : sn;
+    // offsets clamped to the relations: a split whose device wait timed out
+    // (reported as EDPP_ETIMEOUT) leaves offsets that must not steer reads
+    auto clamp = [](int64_t x, int64_t lo, int64_t hi) { return x < lo ? lo : (x > hi ? hi : x); };
+    const int64_t r0 = clamp(rs[b], 0, rn), r1 = clamp(b + 1 < B ? rs[b + 1] : rn, r0, rn);
+    const int64_t s0 = clamp(ss[b], 0, sn), s1 = clamp(b + 1 < B ? ss[b + 1] : sn, s0, sn);
     const bool build_r = (r1 - r0) <= (s1 - s0);
     const uint32_t* bkeys = build_r ? rk : sk;
     const uint32_t* pkeys = build_r ? sk : rk;

@@ -118,6 +118,13 @@ int  edpp_ctx_sync(edpp_ctx* ctx);
 void* edpp_ctx_stream(edpp_ctx* ctx);
 /* Number of compute units and XCDs the context's device reports. */
 int  edpp_ctx_device_info(edpp_ctx* ctx, int* cus, int* xcds, int64_t* l2_bytes, int64_t* mall_bytes);
+/* Polls a device-side wait makes (a look-back over a predecessor tile's
+ * status, a batch phase's counter) before it gives up: the kernel then
+ * finishes with an invalid result and flags EDPP_ETIMEOUT, which the next
+ * edpp_ctx_sync or timed call returns.  Default 2^22 polls (seconds); 0 gives
+ * up at the first wait.  No counterpart in the reference, whose chained scan
+ * spins without a bound (opencl/kernels/scan_global_chain_kernel.cl:51).     */
+int  edpp_ctx_set_watchdog(edpp_ctx* ctx, uint32_t polls);
 const char* edpp_strerror(int status);
 const char* edpp_last_error(edpp_ctx* ctx);   /* message of the last failing call on ctx */
 int  edpp_abi_version(void);
