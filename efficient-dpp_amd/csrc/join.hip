@@ -9,9 +9,10 @@
 //   sum over keys k of |{i: R[i] == k}| * |{j: S[j] == k}|.
 // Here the keys are first replaced by a bijective mix of their bits, so the
 // partitions stay balanced for structured keys; equal keys keep equal mixes.
-// The partition width targets 4096 build keys; a workgroup builds its table
-// from the smaller side of its pair in chunks of at most kJoinChunk keys
-// (load factor <= 1/2), so any skew ends, at the cost of re-probing.  Every
+// The partition width targets 4096 build keys (up to 16 partition bits); a
+// workgroup builds its table from the smaller side of its pair in chunks of at
+// most kJoinChunk keys (load factor <= 1/2), so any skew ends, at the cost of
+// re-probing.  Every
 // key is valid (no sentinel value, unlike the reference's -1).
 #include <algorithm>
 #include "ctx.hpp"
@@ -25,6 +26,11 @@ constexpr int kJoinBlock = 1024;
 constexpr int kJoinSlots = 16384;  // 64-bit LDS slots (128 KiB): (count << 32) | key; count 0 = empty
 constexpr int kJoinChunk = kJoinSlots / 2;  // build keys per table fill
 constexpr int kJoinTargetBuild = 4096;
+constexpr int kJoinPre = kJoinChunk / kJoinBlock;  // keys per lane per side held in registers (8)
+// Partition bits at most: the split runs up to 16 bucket bits as two stable
+// passes, so relations up to 2^28 keys keep about kJoinTargetBuild build keys
+// per partition and one table fill each.
+constexpr int kJoinMaxBits = 16;
 
 // murmur3's 32-bit finalizer: a bijection of 32-bit words
 __host__ __device__ inline uint32_t mix32(uint32_t x) {
@@ -103,7 +109,31 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe(const uint32_t* __res
     const int64_t b0 = build_r ? r0 : s0, b1 = build_r ? r1 : s1;
     const int64_t p0 = build_r ? s0 : r0, p1 = build_r ? s1 : r1;
     unsigned long long local = 0;
-    if (b1 > b0 && p1 > p0) {
+    if (b1 > b0 && p1 > p0 && b1 - b0 <= kJoinPre * kJoinBlock && p1 - p0 <= kJoinPre * kJoinBlock) {
+        // one table fill: both sides' keys are loaded into registers first, so
+        // their global latency overlaps the table clearing (the workgroup
+        // holds the CU's LDS alone, and no other workgroup hides it)
+        uint32_t bk[kJoinPre], pk[kJoinPre];
+        #pragma unroll
+        for (int q = 0; q < kJoinPre; ++q) {
+            const int64_t i = b0 + threadIdx.x + (int64_t)q * kJoinBlock;
+            bk[q] = i < b1 ? bkeys[i] : 0u;
+        }
+        #pragma unroll
+        for (int q = 0; q < kJoinPre; ++q) {
+            const int64_t i = p0 + threadIdx.x + (int64_t)q * kJoinBlock;
+            pk[q] = i < p1 ? pkeys[i] : 0u;
+        }
+        for (int i = threadIdx.x; i < kJoinSlots; i += kJoinBlock) table[i] = 0ull;
+        __syncthreads();
+        #pragma unroll
+        for (int q = 0; q < kJoinPre; ++q)
+            if (b0 + threadIdx.x + (int64_t)q * kJoinBlock < b1) join_insert(table, bk[q], pbits);
+        __syncthreads();
+        #pragma unroll
+        for (int q = 0; q < kJoinPre; ++q)
+            if (p0 + threadIdx.x + (int64_t)q * kJoinBlock < p1) local += join_lookup(table, pk[q], pbits);
+    } else if (b1 > b0 && p1 > p0) {
         for (int64_t c0 = b0; c0 < b1; c0 += kJoinChunk) {
             const int64_t c1 = std::min<int64_t>(c0 + kJoinChunk, b1);
             for (int i = threadIdx.x; i < kJoinSlots; i += kJoinBlock) table[i] = 0ull;
@@ -136,7 +166,7 @@ int launch_join_count(edpp_ctx* c, const uint32_t* rk, int64_t rn, const uint32_
     // partition bits: about kJoinTargetBuild keys of the smaller side per partition
     const int64_t small = std::min(rn, sn);
     int pbits = 0;
-    while (pbits < 12 && (small >> pbits) > kJoinTargetBuild) ++pbits;
+    while (pbits < kJoinMaxBits && (small >> pbits) > kJoinTargetBuild) ++pbits;
     const int B = 1 << pbits;
     const size_t rb = ((size_t)rn * 4 + 255) & ~(size_t)255, sb = ((size_t)sn * 4 + 255) & ~(size_t)255;
     const size_t stb = ((size_t)B * 4 + 255) & ~(size_t)255;

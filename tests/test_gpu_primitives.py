@@ -466,11 +466,12 @@ def test_hash_join_skew_and_special_keys(torch_gpu, ctx, orc):
     assert int(cnt.item()) == orc.join_count(r, s)
 
 
-def test_hash_join_full_size_property(torch_gpu, ctx):
-    # R = a random permutation of [0, 2^26), S = R's first half shifted by a
-    # multiple of 2^26 plus R's second half: exactly 2^25 matching pairs
+@pytest.mark.parametrize("n", [(1 << 25) + 3, 1 << 26])
+def test_hash_join_full_size_property(torch_gpu, ctx, n):
+    # R = a random permutation of [0, n), S = R's first half shifted by n plus
+    # R's second half: exactly n//2 matching pairs.  2^25+3 keys partition by
+    # 13 bits (split passes of 6 and 7 bits), 2^26 by 14 (7 and 7)
     torch = torch_gpu
-    n = 1 << 26
     r = empty(torch, n, torch.int32)
     ctx.gen_perm(r, 77, 0)
     s = r.clone()
