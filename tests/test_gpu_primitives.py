@@ -468,9 +468,9 @@ def test_hash_join_skew_and_special_keys(torch_gpu, ctx, orc):
 
 @pytest.mark.parametrize("n", [(1 << 25) + 3, 1 << 26])
 def test_hash_join_full_size_property(torch_gpu, ctx, n):
-    # R = a random permutation of [0, n), S = R's first half shifted by n plus
-    # R's second half: exactly n//2 matching pairs.  2^25+3 keys partition by
-    # 13 bits (split passes of 6 and 7 bits), 2^26 by 14 (7 and 7)
+    # R = a random permutation of [0, n), S = R's first n//2 keys shifted by n
+    # plus the rest of R: exactly n - n//2 matching pairs.  2^25+3 keys
+    # partition by 13 bits (split passes of 6 and 7 bits), 2^26 by 14 (7 and 7)
     torch = torch_gpu
     r = empty(torch, n, torch.int32)
     ctx.gen_perm(r, 77, 0)
@@ -478,7 +478,7 @@ def test_hash_join_full_size_property(torch_gpu, ctx, n):
     s[: n // 2] += n
     cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
     ctx.join_count(r, s, cnt)
-    assert int(cnt.item()) == n // 2
+    assert int(cnt.item()) == n - n // 2
     del r, s
 
 
