@@ -91,15 +91,58 @@ __device__ uint32_t scan_partials_network(uint32_t* part, int L, int network, ui
 }
 
 // MATRIX: EDPP_MATRIX_* ; the whole tile is n = L*tile elements
+// Coalesced copy of n = L*tile words between global memory and LDS: 16-byte
+// vectors (at most kMaxRow/4 per lane, four loads in flight before their
+// stores) when both sides are 16-byte aligned, else words.
+__device__ __forceinline__ void copy_tile(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, int n,
+                                          bool vec) {
+    const int t = threadIdx.x, L = blockDim.x;
+    if (vec) {
+        const int nv = n / 4;  // L is a power of two >= 64, so 4 divides n
+        const uint4* s4 = reinterpret_cast<const uint4*>(src);
+        uint4* d4 = reinterpret_cast<uint4*>(dst);
+        // four rounds of four vectors: 16 registers of loads in flight
+        #pragma unroll
+        for (int r = 0; r < kMaxRow / 16; ++r) {
+            uint4 v[4];
+            #pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = t + (4 * r + q) * L;
+                v[q] = i < nv ? s4[i] : make_uint4(0u, 0u, 0u, 0u);
+            }
+            #pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = t + (4 * r + q) * L;
+                if (i < nv) d4[i] = v[q];
+            }
+        }
+    } else {
+        for (int i = t; i < n; i += L) dst[i] = src[i];
+    }
+}
+
+// Row-wise copy for a padded LDS layout (row stride S = tile + 1, odd, so the
+// lanes' rows fall in different banks): wave w moves rows w, w+W, ..., lane l
+// element l of the row (tile <= 64).  to_lds: dst[r*S + l] = src[r*tile + l];
+// else dst[r*tile + l] = src[r*S + l].
+__device__ __forceinline__ void copy_rows(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, int tile,
+                                          int S, bool to_lds) {
+    const int W = blockDim.x / kWave, w = threadIdx.x / kWave, l = lane_id();
+    const int L = blockDim.x;
+    if (l >= tile) return;
+    const int ss = to_lds ? tile : S, ds = to_lds ? S : tile;
+    for (int r = w; r < L; r += W) dst[r * ds + l] = src[r * ss + l];
+}
+
 template <int MATRIX>
 __global__ __launch_bounds__(kMaxLocal) void k_matrix_scan(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                           int tile, int network) {
-    extern __shared__ uint32_t lds[];
+                                                           int tile, int network, bool vec, int S) {
+    extern __shared__ __align__(16) uint32_t lds[];
     const int L = blockDim.x, t = threadIdx.x;
     const int n = L * tile;
     uint32_t* part = lds;          // [L]
     uint32_t* wtmp = part + L;     // [16]
-    uint32_t* data = wtmp + 16;    // [n] for the LDS-staged variants
+    uint32_t* data = wtmp + 16;    // [n] for the LDS-staged variants (16-byte aligned: L >= 64)
     if (MATRIX == EDPP_MATRIX_LM_SERIAL) {
         if (t == 0) {
             uint32_t acc = 0;
@@ -119,12 +162,13 @@ __global__ __launch_bounds__(kMaxLocal) void k_matrix_scan(const uint32_t* __res
         for (int k = 0; k < kMaxRow; ++k)
             if (k < tile) { reg[k] = in[t * tile + k]; s += reg[k]; }
     } else {
-        // coalesced copy into LDS, then rows out of LDS
-        for (int i = t; i < n; i += L) data[i] = in[i];
+        // coalesced copy into LDS (rows of stride S), then rows out of LDS
+        if (S == tile) copy_tile(in, data, n, vec);
+        else copy_rows(in, data, tile, S, true);
         __syncthreads();
         #pragma unroll
         for (int k = 0; k < kMaxRow; ++k)
-            if (k < tile) { reg[k] = data[t * tile + k]; s += reg[k]; }
+            if (k < tile) { reg[k] = data[t * S + k]; s += reg[k]; }
     }
     part[t] = s;
     __syncthreads();
@@ -134,9 +178,10 @@ __global__ __launch_bounds__(kMaxLocal) void k_matrix_scan(const uint32_t* __res
         __syncthreads();
         #pragma unroll
         for (int k = 0; k < kMaxRow; ++k)
-            if (k < tile) { data[t * tile + k] = acc; acc += reg[k]; }
+            if (k < tile) { data[t * S + k] = acc; acc += reg[k]; }
         __syncthreads();
-        for (int i = t; i < n; i += L) out[i] = data[i];
+        if (S == tile) copy_tile(data, out, n, vec);
+        else copy_rows(data, out, tile, S, false);
     } else {
         // rows written from registers (the reference's LM and REG final scans)
         #pragma unroll
@@ -155,30 +200,36 @@ int launch_scan_local(edpp_ctx* c, const int32_t* in, int32_t* out, int tile, in
         return fail(c, EDPP_EINVAL, "unknown network %d", network);
     const size_t n = (size_t)local_size * tile;
     size_t lds = (size_t)(local_size + 16) * 4;
-    if (matrix_type == EDPP_MATRIX_LM || matrix_type == EDPP_MATRIX_LM_REG) lds += n * 4;
+    // LDS rows of an even tile are padded to an odd stride where that still fits
+    int S = tile;
+    if (matrix_type == EDPP_MATRIX_LM || matrix_type == EDPP_MATRIX_LM_REG) {
+        if ((tile & 1) == 0 && lds + (size_t)local_size * (tile + 1) * 4 <= 160 * 1024) S = tile + 1;
+        lds += (size_t)local_size * S * 4;
+    }
     if (lds > 160 * 1024) return fail(c, EDPP_EINVAL, "tile of %zu elements does not fit in LDS", n);
     const uint32_t* a = (const uint32_t*)in;
     uint32_t* b = (uint32_t*)out;
+    const bool vec = aligned16(a) && aligned16(b);
     switch (matrix_type) {
         case EDPP_MATRIX_LM:
             EDPP_HIP(c, hipFuncSetAttribute((const void*)k_matrix_scan<EDPP_MATRIX_LM>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             hipLaunchKernelGGL(k_matrix_scan<EDPP_MATRIX_LM>, dim3(1), dim3(local_size), lds, c->stream, a, b, tile,
-                               network);
+                               network, vec, S);
             break;
         case EDPP_MATRIX_REG:
             hipLaunchKernelGGL(k_matrix_scan<EDPP_MATRIX_REG>, dim3(1), dim3(local_size), lds, c->stream, a, b, tile,
-                               network);
+                               network, vec, S);
             break;
         case EDPP_MATRIX_LM_REG:
             EDPP_HIP(c, hipFuncSetAttribute((const void*)k_matrix_scan<EDPP_MATRIX_LM_REG>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             hipLaunchKernelGGL(k_matrix_scan<EDPP_MATRIX_LM_REG>, dim3(1), dim3(local_size), lds, c->stream, a, b,
-                               tile, network);
+                               tile, network, vec, S);
             break;
         case EDPP_MATRIX_LM_SERIAL:
             hipLaunchKernelGGL(k_matrix_scan<EDPP_MATRIX_LM_SERIAL>, dim3(1), dim3(local_size), lds, c->stream, a, b,
-                               tile, network);
+                               tile, network, vec, S);
             break;
         default:
             return fail(c, EDPP_EINVAL, "unknown matrix scan type %d", matrix_type);

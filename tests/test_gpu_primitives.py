@@ -147,14 +147,33 @@ def test_scan_full_size_properties(torch_gpu, ctx):
 # ---------------------------------------------------------------- local (matrix) scans
 @pytest.mark.parametrize("matrix", [0, 1, 2, 3])
 @pytest.mark.parametrize("network", [0, 1, 2, 3])
-@pytest.mark.parametrize("tile,local", [(1, 512), (11, 512), (49, 512), (64, 64), (16, 1024), (3, 128)])
+@pytest.mark.parametrize("tile,local", [(1, 512), (11, 512), (49, 512), (64, 64), (16, 1024), (3, 128), (2, 512),
+                                        (64, 512), (38, 1024)])
 def test_scan_local(torch_gpu, ctx, orc, matrix, network, tile, local):
+    # even tiles take LDS rows padded to an odd stride, except (38, 1024), whose
+    # padded rows would not fit in LDS
     torch = torch_gpu
     n = tile * local
     x = orc.gen_i32(n, tile * 1000 + local, 15)
     out = empty(torch, n, torch.int32)
     ctx.scan_local(dev(torch, x), out, tile, local, matrix, network)
     assert np.array_equal(host(out), orc.scan_exclusive(x)[0])
+
+
+@pytest.mark.parametrize("matrix", [0, 2])
+@pytest.mark.parametrize("tile", [11, 16])
+def test_scan_local_unaligned(torch_gpu, ctx, orc, matrix, tile):
+    # input and output one word past a 16-byte boundary: word-wise copies
+    torch = torch_gpu
+    local = 256
+    n = tile * local
+    x = orc.gen_i32(n, 4242 + tile, 15)
+    xin = torch.zeros(n + 1, dtype=torch.int32, device="cuda")
+    xin[1:] = dev(torch, x)
+    out = torch.full((n + 1,), -7, dtype=torch.int32, device="cuda")
+    ctx.scan_local(xin[1:], out[1:], tile, local, matrix, 3)
+    assert int(out[0].item()) == -7
+    assert np.array_equal(host(out[1:]), orc.scan_exclusive(x)[0])
 
 
 # ---------------------------------------------------------------- gather / scatter
