@@ -359,6 +359,24 @@ def test_split_i64(torch_gpu, ctx, orc, strategy, B):
     assert np.array_equal(host(kout), ko) and np.array_equal(host(vout), vo) and np.array_equal(host(st), start)
 
 
+@pytest.mark.parametrize("B", [2, 16, 256])
+def test_split_unaligned_input(torch_gpu, ctx, orc, B):
+    # keys and values one word past a 16-byte boundary
+    torch = torch_gpu
+    n = 100_003
+    keys, vals = orc.gen_split_keys(n, B, 1, 900 + B)
+    ko, vo, start = orc.split(keys, B, vals)
+    dk = torch.zeros(n + 1, dtype=torch.int32, device="cuda")
+    dvv = torch.zeros(n + 1, dtype=torch.int32, device="cuda")
+    dk[1:] = dev(torch, keys)
+    dvv[1:] = dev(torch, vals)
+    kout = empty(torch, n, torch.int32)
+    vout = empty(torch, n, torch.int32)
+    st = empty(torch, B, torch.int32)
+    ctx.split(dk[1:], kout, B, dvv[1:], vout, st, n, 1)
+    assert np.array_equal(host(kout), ko) and np.array_equal(host(vout), vo) and np.array_equal(host(st), start)
+
+
 @pytest.mark.parametrize("B", [2, 16, 256, 4096])
 def test_split_full_size_properties(torch_gpu, ctx, B):
     # 2^28 pairs, Zipf keys, vals = original index: bucket ids non-decreasing,
