@@ -442,11 +442,13 @@ def suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs):
     # STREAM (opencl/test/test_bandwidth.cpp: 2^28 int32 per array)
     b = torch.empty(n, dtype=i32, device="cuda")
     ctx.gen_i32(b, 0x11)
-    st = {}
+    st, st_ms = {}, {}
     for name, op, nb in [("copy", 0, 8), ("scale", 1, 8), ("add", 2, 12), ("triad", 3, 12)]:
         t = T.time(lambda: ctx.stream_op(op, src, b, dst, n), reps=20)
         st[name] = round(gbs(nb * n, t["hampel_ms"]), 2)
+        st_ms[name] = {k: round(v, 4) for k, v in t.items()}
     res["stream_gbs"] = st
+    res["stream_ms"] = st_ms
     print("stream", st, file=sys.stderr, flush=True)
     # scans: 2^24 (values in [0,15]) and 2^28; warm (back to back) and cold
     # (the 256 MiB Infinity Cache evicted by a 1 GiB write before each run):
@@ -479,7 +481,9 @@ def suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs):
             tg = T.time(lambda: ctx.gather(src, dst, idx, n, p), reps=10)
             ts = T.time(lambda: ctx.scatter(src, dst, idx, n, p), reps=10)
             gs[f"{kname}_p{p}"] = {"gather_gbs": round(gbs(12.0 * n, tg["hampel_ms"]), 2),
-                                   "scatter_gbs": round(gbs(12.0 * n, ts["hampel_ms"]), 2)}
+                                   "scatter_gbs": round(gbs(12.0 * n, ts["hampel_ms"]), 2),
+                                   "gather_ms": {k: round(v, 4) for k, v in tg.items()},
+                                   "scatter_ms": {k: round(v, 4) for k, v in ts.items()}}
     ctx.gen_perm(idx, 0x5EED, edpp.PERM_RANDOM)
     res["gather_scatter"] = gs
     print("gather_scatter", gs, file=sys.stderr, flush=True)
@@ -494,6 +498,7 @@ def suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs):
             t = T.time(lambda: ctx.split(src, ko, B, b, vo, start, n, edpp.SPLIT_AUTO), reps=20)
             g = gbs(16.0 * n, t["hampel_ms"])
             sp[f"B{B}_{dname}"] = {"gbs": round(g, 2), "ms": round(t["hampel_ms"], 3),
+                                   "median_ms": round(t["median_ms"], 3), "min_ms": round(t["min_ms"], 3),
                                    "frac_copy": round(g / copy_gbs, 4)}
             print("split", B, dname, sp[f"B{B}_{dname}"], file=sys.stderr, flush=True)
     res["split_kv"] = sp
@@ -532,7 +537,8 @@ def suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs):
                      ("keys_u32", lambda: ctx.sort(src, ko, None, None, n, 0, 32))]:
         t = T.time(fn, reps=10)
         per = 16.0 if name.startswith("pairs") else 8.0  # one read and one write of each pair
-        srt[name] = {"ms": round(t["hampel_ms"], 3), "gkeys_s": round(n / (t["hampel_ms"] * 1e-3) / 1e9, 2),
+        srt[name] = {"ms": round(t["hampel_ms"], 3), "median_ms": round(t["median_ms"], 3),
+                     "min_ms": round(t["min_ms"], 3), "gkeys_s": round(n / (t["hampel_ms"] * 1e-3) / 1e9, 2),
                      "gbs_one_pass_bytes": round(gbs(per * n, t["hampel_ms"]), 1)}
     res["sort"] = srt
     print("sort", srt, file=sys.stderr, flush=True)
@@ -545,7 +551,8 @@ def suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs):
         ctx.gen_perm(vo[:m], 0x1B, edpp.PERM_RANDOM, n=m)
         t = T.time(lambda: ctx.join_count(ko[:m], vo[:m], cnt, m, m), reps=5)
         assert int(cnt.item()) == m, "join count mismatch"
-        jn[f"2^{lg}x2^{lg}"] = {"ms": round(t["hampel_ms"], 3),
+        jn[f"2^{lg}x2^{lg}"] = {"ms": round(t["hampel_ms"], 3), "median_ms": round(t["median_ms"], 3),
+                                "min_ms": round(t["min_ms"], 3),
                                 "g_tuples_s": round(2 * m / (t["hampel_ms"] * 1e-3) / 1e9, 2)}
     res["hash_join"] = jn
     print("hash_join", jn, file=sys.stderr, flush=True)
