@@ -52,6 +52,7 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=1 << 26, help="elements in the CPU baseline sample")
     p.add_argument("--passes", type=int, default=0, help="pass count of the default step (0 = library schedule)")
     p.add_argument("--out", default=None, help="also write the JSON line to this file")
+    p.add_argument("--buckets", type=int, default=256, help="bucket count of --profile split")
     p.add_argument("--profile", default=None,
                    choices=["stream", "gather", "scatter", "gather_local", "scatter_local", "scan", "split", "sort",
                             "join", "batched", "mlp"],
@@ -258,9 +259,9 @@ def profile_only(args, ctx, edpp, torch):
     elif args.profile == "split":
         v = torch.empty(n, dtype=i32, device="cuda")
         vo = torch.empty(n, dtype=i32, device="cuda")
-        ctx.gen_split_keys(a, v, 256, edpp.KEYS_UNIFORM, 0x5917)
+        ctx.gen_split_keys(a, v, args.buckets, edpp.KEYS_UNIFORM, 0x5917)
         for _ in range(reps):
-            ctx.split(a, b, 256, v, vo, None, n, edpp.SPLIT_AUTO)
+            ctx.split(a, b, args.buckets, v, vo, None, n, edpp.SPLIT_AUTO)
     elif args.profile == "sort":
         v = torch.arange(n, dtype=i32, device="cuda")
         vo = torch.empty(n, dtype=i32, device="cuda")
