@@ -557,6 +557,32 @@ def suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs):
                                 "g_tuples_s": round(2 * m / (t["hampel_ms"] * 1e-3) / 1e9, 2)}
     res["hash_join"] = jn
     print("hash_join", jn, file=sys.stderr, flush=True)
+    # 64-bit elements (int64 keys and values, int32 index and split values):
+    # 2^27 elements, so each 64-bit array is 1 GiB as at 2^28 int32
+    m = 1 << 27
+    i64 = torch.int64
+    wa = src.view(i64)[:m]
+    wb = dst.view(i64)[:m]
+    ctx.gen_i64(wa, 0x6464, 15)
+    wide = {}
+    t = T.time(lambda: ctx.scan(wa, wb, m, edpp.SCAN_LOOKBACK, wide=True), reps=20)
+    wide["scan_lookback"] = {"gbs": round(gbs(16.0 * m, t["hampel_ms"]), 2), "ms": round(t["hampel_ms"], 4)}
+    widx = idx[:m]
+    ctx.gen_perm(widx, 0x5EED, edpp.PERM_RANDOM, n=m)
+    for p in (1, 4):
+        tg = T.time(lambda: ctx.gather(wa, wb, widx, m, p, wide=True), reps=10)
+        ts = T.time(lambda: ctx.scatter(wa, wb, widx, m, p, wide=True), reps=10)
+        wide[f"random_p{p}"] = {"gather_gbs": round(gbs(20.0 * m, tg["hampel_ms"]), 2),
+                                "scatter_gbs": round(gbs(20.0 * m, ts["hampel_ms"]), 2)}
+    ctx.gen_perm(idx, 0x5EED, edpp.PERM_RANDOM)
+    kv = b[:m]
+    kvo = vo[:m]
+    for B in (16, 256):
+        ctx.gen_split_keys(wa, kv, B, 0, 0x5917 + B, wide=True)
+        t = T.time(lambda: ctx.split(wa, wb, B, kv, kvo, start, m, edpp.SPLIT_AUTO, wide=True), reps=10)
+        wide[f"split_B{B}"] = {"gbs": round(gbs(24.0 * m, t["hampel_ms"]), 2), "ms": round(t["hampel_ms"], 3)}
+    res["int64_2p27"] = wide
+    print("int64", wide, file=sys.stderr, flush=True)
     del ko, vo, b
     ctx.gen_i32(src, 0xDA7A)
     return res
