@@ -49,7 +49,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-traffic", action="store_true",
                    help="skip the two rocprofv3 --pmc child runs that measure HBM bytes per launch")
-    p.add_argument("--cpu-sample", type=int, default=1 << 26, help="elements in the CPU baseline sample")
+    p.add_argument("--cpu-sample", type=int, default=1 << 28,
+                   help="elements of the CPU baseline's gather and scatter (default: the full workload)")
     p.add_argument("--passes", type=int, default=0, help="pass count of the default step (0 = library schedule)")
     p.add_argument("--out", default=None, help="also write the JSON line to this file")
     p.add_argument("--buckets", type=int, default=256, help="bucket count of --profile split")
@@ -121,7 +122,7 @@ def cpu_baseline(args, torch_unused=None):
     g(), s()  # first touch, discarded (the reference discards run 0 in its bandwidth tests)
     times = []
     t_end = time.time() + 20.0
-    while len(times) < 5 or (time.time() < t_end and len(times) < 20):
+    while len(times) < 5 or (time.time() < t_end and len(times) < 40):
         times.append(g() + s())
         if len(times) >= 5 and time.time() > t_end:
             break
