@@ -161,6 +161,34 @@ def _ptr(x):
     return x.data_ptr()
 
 
+def _buf(x, count, esize, what):
+    """Device address of buffer `x` once it is known to hold `count` elements
+    of `esize` bytes: a contiguous GPU tensor of that element width and at
+    least that many elements.  A wrong buffer would hand a host or
+    out-of-range address to a kernel, so every mismatch raises ValueError
+    before the library is called.  None stays None (an optional buffer) and
+    a plain integer is taken as a raw device address the caller vouches for.
+    """
+    if x is None or isinstance(x, int):
+        return x
+    if not getattr(x, "is_cuda", False):
+        raise ValueError(f"{what}: not a GPU tensor (device {getattr(x, 'device', type(x).__name__)})")
+    if not x.is_contiguous():
+        raise ValueError(f"{what}: not contiguous")
+    if esize is not None and x.element_size() != esize:
+        raise ValueError(f"{what}: {x.element_size()}-byte elements where {esize}-byte ones are needed")
+    if count is not None and x.numel() < count:
+        raise ValueError(f"{what}: {x.numel()} elements where {count} are needed")
+    return x.data_ptr()
+
+
+def _count(n, what):
+    n = int(n)
+    if n < 0:
+        raise ValueError(f"{what}: n={n} < 0")
+    return n
+
+
 def _numel(x):
     return x.numel() if hasattr(x, "numel") else None
 
@@ -254,7 +282,8 @@ class Context:
         return k.value
 
     def call_features(self, call, d_features):
-        self._check(self._L.edpp_call_features(self.h, ctypes.byref(call), _ptr(d_features)))
+        self._check(self._L.edpp_call_features(self.h, ctypes.byref(call),
+                                               _buf(d_features, NUM_FEATURES, 4, "features")))
 
     def set_model(self, mlp):
         self._check(self._L.edpp_ctx_set_model(self.h, mlp.h if mlp is not None else None))
@@ -271,44 +300,58 @@ class Context:
     def stream(self):
         return self._L.edpp_ctx_stream(self.h)
 
-    # -- primitives (n defaults to the element count of the first buffer)
+    # -- primitives (n defaults to the element count of the first buffer).
+    # Every tensor is checked by _buf before the call: on the GPU, contiguous,
+    # of the element width the entry point reads, and long enough for n.
     def stream_op(self, op, a, b, c, n=None, timed=False):
-        n = _numel(a) if n is None else n
-        return self._call("edpp_stream_i32", int(op), _ptr(a), _ptr(b), _ptr(c), n, timed=timed)
+        n = _count(_numel(a) if n is None else n, "stream")
+        return self._call("edpp_stream_i32", int(op), _buf(a, n, 4, "stream a"), _buf(b, n, 4, "stream b"),
+                          _buf(c, n, 4, "stream out"), n, timed=timed)
 
     def gather(self, src, dst, idx, n=None, passes=0, timed=False, wide=False):
-        n = _numel(idx) if n is None else n
+        n = _count(_numel(idx) if n is None else n, "gather")
         name = "edpp_gather_i64" if wide else "edpp_gather_i32"
-        return self._call(name, _ptr(src), _ptr(dst), _ptr(idx), n, int(passes), timed=timed)
+        w = 8 if wide else 4
+        # src is read at positions idx[i] < its length, which the kernel checks
+        # against n: it must hold n elements as well
+        return self._call(name, _buf(src, n, w, "gather src"), _buf(dst, n, w, "gather dst"),
+                          _buf(idx, n, 4, "gather idx"), n, int(passes), timed=timed)
 
     def scatter(self, src, dst, idx, n=None, passes=0, timed=False, wide=False):
-        n = _numel(idx) if n is None else n
+        n = _count(_numel(idx) if n is None else n, "scatter")
         name = "edpp_scatter_i64" if wide else "edpp_scatter_i32"
-        return self._call(name, _ptr(src), _ptr(dst), _ptr(idx), n, int(passes), timed=timed)
+        w = 8 if wide else 4
+        return self._call(name, _buf(src, n, w, "scatter src"), _buf(dst, n, w, "scatter dst"),
+                          _buf(idx, n, 4, "scatter idx"), n, int(passes), timed=timed)
 
     def sort(self, keys_in, keys_out, vals_in=None, vals_out=None, n=None, begin_bit=0, end_bit=32, signed=False,
              timed=False):
         """Stable LSD radix sort of 32-bit keys by bits [begin_bit, end_bit); values follow."""
-        n = _numel(keys_in) if n is None else n
+        n = _count(_numel(keys_in) if n is None else n, "sort")
         name = "edpp_sort_i32" if signed else "edpp_sort_u32"
-        return self._call(name, _ptr(keys_in), _ptr(keys_out), _ptr(vals_in), _ptr(vals_out), n, int(begin_bit),
-                          int(end_bit), timed=timed)
+        return self._call(name, _buf(keys_in, n, 4, "sort keys_in"), _buf(keys_out, n, 4, "sort keys_out"),
+                          _buf(vals_in, n, 4, "sort vals_in"), _buf(vals_out, n, 4, "sort vals_out"), n,
+                          int(begin_bit), int(end_bit), timed=timed)
 
     def access_scale(self, pattern, src, dst, repeat, n=None, timed=False):
         """dst = 3 * src through the pattern's lane mapping (n a multiple of 64*repeat)."""
-        n = _numel(src) if n is None else n
-        return self._call("edpp_access_scale_i32", int(pattern), _ptr(src), _ptr(dst), n, int(repeat), timed=timed)
+        n = _count(_numel(src) if n is None else n, "access_scale")
+        return self._call("edpp_access_scale_i32", int(pattern), _buf(src, n, 4, "access src"),
+                          _buf(dst, n, 4, "access dst"), n, int(repeat), timed=timed)
 
     def access_chunks(self, src, chunk, order, count, n=None, timed=False):
         """count[0] = odd elements of src, read in chunks taken in `order`."""
-        n = _numel(src) if n is None else n
-        return self._call("edpp_access_chunks_i32", _ptr(src), n, int(chunk), _ptr(order), _ptr(count), timed=timed)
+        n = _count(_numel(src) if n is None else n, "access_chunks")
+        chunks = -(-n // int(chunk)) if int(chunk) > 0 else None
+        return self._call("edpp_access_chunks_i32", _buf(src, n, 4, "access src"), n, int(chunk),
+                          _buf(order, chunks, 4, "access order"), _buf(count, 1, 8, "access count"), timed=timed)
 
     def join_count(self, r_keys, s_keys, count, r_n=None, s_n=None, timed=False):
         """count[0] (uint64/int64 device tensor) = number of equal-key pairs of R and S."""
-        r_n = _numel(r_keys) if r_n is None else r_n
-        s_n = _numel(s_keys) if s_n is None else s_n
-        return self._call("edpp_hash_join_count", _ptr(r_keys), r_n, _ptr(s_keys), s_n, _ptr(count), timed=timed)
+        r_n = _count(_numel(r_keys) if r_n is None else r_n, "join R")
+        s_n = _count(_numel(s_keys) if s_n is None else s_n, "join S")
+        return self._call("edpp_hash_join_count", _buf(r_keys, r_n, 4, "join R"), r_n, _buf(s_keys, s_n, 4, "join S"),
+                          s_n, _buf(count, 1, 8, "join count"), timed=timed)
 
     def schedule_passes(self, n, scatter=False, wide=False):
         """Pass count a passes=0 gather/scatter of n elements uses without a cost model."""
@@ -318,44 +361,58 @@ class Context:
         return out.value
 
     def scan(self, src, dst, n=None, scheme=SCAN_AUTO, total=None, timed=False, wide=False):
-        n = _numel(src) if n is None else n
+        n = _count(_numel(src) if n is None else n, "scan")
         name = "edpp_scan_exclusive_i64" if wide else "edpp_scan_exclusive_i32"
-        return self._call(name, _ptr(src), _ptr(dst), n, int(scheme), _ptr(total), timed=timed)
+        w = 8 if wide else 4
+        return self._call(name, _buf(src, n, w, "scan in"), _buf(dst, n, w, "scan out"), n, int(scheme),
+                          _buf(total, 1, w, "scan total"), timed=timed)
 
     def scan_local(self, src, dst, tile_size, local_size=512, matrix=MATRIX_LM_REG, network=NET_WAVE, timed=False):
-        return self._call("edpp_scan_local_i32", _ptr(src), _ptr(dst), int(tile_size), int(local_size), int(matrix),
-                          int(network), timed=timed)
+        n = max(int(tile_size), 0) * max(int(local_size), 0)
+        return self._call("edpp_scan_local_i32", _buf(src, n, 4, "scan_local in"), _buf(dst, n, 4, "scan_local out"),
+                          int(tile_size), int(local_size), int(matrix), int(network), timed=timed)
 
     def split(self, keys_in, keys_out, buckets, vals_in=None, vals_out=None, start=None, n=None,
               strategy=SPLIT_AUTO, timed=False, wide=False):
-        n = _numel(keys_in) if n is None else n
+        n = _count(_numel(keys_in) if n is None else n, "split")
         name = "edpp_split_i64" if wide else "edpp_split_i32"
-        return self._call(name, _ptr(keys_in), _ptr(keys_out), _ptr(vals_in), _ptr(vals_out), _ptr(start), n,
-                          int(buckets), int(strategy), timed=timed)
+        w = 8 if wide else 4
+        return self._call(name, _buf(keys_in, n, w, "split keys_in"), _buf(keys_out, n, w, "split keys_out"),
+                          _buf(vals_in, n, 4, "split vals_in"), _buf(vals_out, n, 4, "split vals_out"),
+                          _buf(start, max(int(buckets), 0), 4, "split start"), n, int(buckets), int(strategy),
+                          timed=timed)
 
     # -- reproducible inputs
     def gen_i32(self, out, seed, mask=-1, n=None):
-        self._check(self._L.edpp_gen_i32(self.h, _ptr(out), _numel(out) if n is None else n, seed & (2**64 - 1),
-                                         mask))
+        n = _count(_numel(out) if n is None else n, "gen_i32")
+        self._check(self._L.edpp_gen_i32(self.h, _buf(out, n, 4, "gen out"), n, seed & (2**64 - 1), mask))
 
     def gen_i64(self, out, seed, mask=-1, n=None):
-        self._check(self._L.edpp_gen_i64(self.h, _ptr(out), _numel(out) if n is None else n, seed & (2**64 - 1),
-                                         mask))
+        n = _count(_numel(out) if n is None else n, "gen_i64")
+        self._check(self._L.edpp_gen_i64(self.h, _buf(out, n, 8, "gen out"), n, seed & (2**64 - 1), mask))
 
     def gen_perm(self, idx, seed, kind=PERM_RANDOM, n=None):
-        self._check(self._L.edpp_gen_perm(self.h, _ptr(idx), _numel(idx) if n is None else n, seed & (2**64 - 1),
-                                          int(kind)))
+        n = _count(_numel(idx) if n is None else n, "gen_perm")
+        self._check(self._L.edpp_gen_perm(self.h, _buf(idx, n, 4, "gen_perm idx"), n, seed & (2**64 - 1), int(kind)))
 
     def gen_split_keys(self, keys, vals, buckets, dist, seed, n=None, wide=False):
+        n = _count(_numel(keys) if n is None else n, "gen_split_keys")
         name = "edpp_gen_split_keys_i64" if wide else "edpp_gen_split_keys_i32"
-        self._check(getattr(self._L, name)(self.h, _ptr(keys), _ptr(vals), _numel(keys) if n is None else n,
-                                           int(buckets), int(dist), seed & (2**64 - 1)))
+        self._check(getattr(self._L, name)(self.h, _buf(keys, n, 8 if wide else 4, "gen keys"),
+                                           _buf(vals, n, 4, "gen vals"), n, int(buckets), int(dist),
+                                           seed & (2**64 - 1)))
 
 
 def make_job(kind, n, src, dst, elem_bytes=4, idx=None, buckets=0, vals_in=None, vals_out=None, aux=None):
-    """One batch job descriptor; buffers as for the single-call methods."""
-    return Job(int(kind), int(elem_bytes), int(n), int(buckets), 0, _ptr(src), _ptr(dst), _ptr(idx), _ptr(vals_in),
-               _ptr(vals_out), _ptr(aux))
+    """One batch job descriptor; buffers as for the single-call methods and
+    checked the same way (aux: the scan total, or the split's bucket starts)."""
+    n = _count(n, "job")
+    w = int(elem_bytes) if int(elem_bytes) in (4, 8) else None
+    aux_count, aux_w = (1, w) if int(kind) == JOB_SCAN else ((max(int(buckets), 0), 4) if int(kind) == JOB_SPLIT
+                                                             else (None, None))
+    return Job(int(kind), int(elem_bytes), n, int(buckets), 0, _buf(src, n, w, "job in"), _buf(dst, n, w, "job out"),
+               _buf(idx, n, 4, "job idx"), _buf(vals_in, n, 4, "job vals_in"), _buf(vals_out, n, 4, "job vals_out"),
+               _buf(aux, aux_count, aux_w, "job aux"))
 
 
 class Batch:
@@ -401,8 +458,12 @@ class Batch:
 
 
 def make_call(op, n, src, dst, elem_bytes=4, idx=None, buckets=0, vals_in=None, vals_out=None):
-    return Call(int(op), int(elem_bytes), int(n), int(buckets), 0, _ptr(src), _ptr(dst), _ptr(idx), _ptr(vals_in),
-                _ptr(vals_out))
+    """One call descriptor for the autotuner and the cost model, its buffers
+    checked as for the single-call methods."""
+    n = _count(n, "call")
+    w = int(elem_bytes) if int(elem_bytes) in (4, 8) else None
+    return Call(int(op), int(elem_bytes), n, int(buckets), 0, _buf(src, n, w, "call in"), _buf(dst, n, w, "call out"),
+                _buf(idx, n, 4, "call idx"), _buf(vals_in, n, 4, "call vals_in"), _buf(vals_out, n, 4, "call vals_out"))
 
 
 def candidates():
@@ -440,17 +501,25 @@ class MLP:
 
     def forward(self, x, mask, probs=None, choice=None):
         rows = x.shape[0]
-        self.ctx._check(self.ctx._L.edpp_mlp_forward(self.h, _ptr(x), _ptr(mask), rows, _ptr(probs), _ptr(choice)))
+        self.ctx._check(self.ctx._L.edpp_mlp_forward(
+            self.h, _buf(x, rows * NUM_FEATURES, 4, "mlp x"), _buf(mask, rows * MAX_CANDIDATES, 4, "mlp mask"), rows,
+            _buf(probs, rows * MAX_CANDIDATES, 4, "mlp probs"), _buf(choice, rows, 4, "mlp choice")))
 
     def loss_grad(self, x, target, mask, loss, grad):
-        self.ctx._check(self.ctx._L.edpp_mlp_loss_grad(self.h, _ptr(x), _ptr(target), _ptr(mask), x.shape[0],
-                                                       _ptr(loss), _ptr(grad)))
+        rows = x.shape[0]
+        self.ctx._check(self.ctx._L.edpp_mlp_loss_grad(
+            self.h, _buf(x, rows * NUM_FEATURES, 4, "mlp x"), _buf(target, rows * MAX_CANDIDATES, 4, "mlp target"),
+            _buf(mask, rows * MAX_CANDIDATES, 4, "mlp mask"), rows, _buf(loss, 1, 4, "mlp loss"),
+            _buf(grad, self.nparams, 4, "mlp grad")))
 
     def train(self, x, target, mask, epochs, lr=1e-2):
         import numpy as np
         losses = np.zeros(max(epochs, 1), np.float32)
-        self.ctx._check(self.ctx._L.edpp_mlp_train(self.h, _ptr(x), _ptr(target), _ptr(mask), x.shape[0],
-                                                   int(epochs), float(lr), losses.ctypes.data_as(_c_fp)))
+        rows = x.shape[0]
+        self.ctx._check(self.ctx._L.edpp_mlp_train(
+            self.h, _buf(x, rows * NUM_FEATURES, 4, "mlp x"), _buf(target, rows * MAX_CANDIDATES, 4, "mlp target"),
+            _buf(mask, rows * MAX_CANDIDATES, 4, "mlp mask"), rows, int(epochs), float(lr),
+            losses.ctypes.data_as(_c_fp)))
         return losses[:epochs]
 
     def memo(self, enabled=True):

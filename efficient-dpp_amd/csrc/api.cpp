@@ -390,7 +390,7 @@ int edpp_default_passes(edpp_ctx* c, int64_t n, int elem_bytes, int scatter, int
 template <class T>
 static int scan_common(edpp_ctx* c, const T* in, T* out, int64_t n, int scheme, T* d_total, double* ms) {
     EDPP_CHECK_CTX(c);
-    if (n < 0 || n >= (1ll << 40)) return fail(c, EDPP_EINVAL, "n=%lld outside [0, 2^40)", (long long)n);
+    if (n < 0 || n >= (1ll << 31)) return fail(c, EDPP_EINVAL, "n=%lld outside [0, 2^31)", (long long)n);
     if (n > 0 && (!in || !out)) return fail(c, EDPP_EINVAL, "null buffer");
     if (scheme < EDPP_SCAN_AUTO || scheme > EDPP_SCAN_SSA) return fail(c, EDPP_EINVAL, "unknown scheme %d", scheme);
     int rc = timing_begin(c, ms);

@@ -57,11 +57,17 @@ __host__ __device__ inline int64_t tile_elems(int kind, int wide) {
 }
 constexpr int64_t kHistScanTile = 4096;  // TileScan<uint32_t,256,4>
 
-// Cross-workgroup data of one launch (histograms, counters) only moves through
+// Cross-workgroup data of one launch (histograms, counters) moves through
 // agent-scope atomic loads and stores, which stay coherent across the XCDs'
-// L2s without fences; polling is relaxed, so no spin loop invalidates a cache.
+// L2s.  A finished tile is counted with a release increment and a waiter
+// that has seen its count takes one acquire fence, so the counts written
+// before the increment are ordered before the waiter's reads of them by the
+// memory model, not only by the hardware.  Polling stays relaxed: no spin
+// iteration invalidates a cache, only the one successful wait does.
 
-// thread 0 waits for *p >= target
+// thread 0 waits for *p >= target, then acquires (agent scope) what the
+// release increments up to target published; the barrier passes that on to
+// the workgroup
 __device__ bool wait_count(uint32_t* p, uint32_t target, uint32_t spin_limit, int32_t* err, uint32_t* s_ok) {
     if (threadIdx.x == 0) {
         uint32_t spins = 0, ok = 1;
@@ -73,17 +79,19 @@ __device__ bool wait_count(uint32_t* p, uint32_t target, uint32_t spin_limit, in
             }
             __builtin_amdgcn_s_sleep(2);
         }
+        if (ok) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         *s_ok = ok;
     }
     __syncthreads();
     return *s_ok != 0;
 }
 
-// every lane's atomic stores have completed before thread 0 counts this tile
+// every lane's stores of this tile complete and reach thread 0 through the
+// barrier; thread 0 then counts the tile with a release increment
 __device__ void signal_count(uint32_t* p) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (threadIdx.x == 0) fetch_add(p, 1u);
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------- scan tiles

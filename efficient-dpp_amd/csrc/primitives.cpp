@@ -107,7 +107,8 @@ double WG_split(const int* d_in, int* d_out, int* d_start, int length, int bucke
                 DataStruc structure, const int* d_in_values, int* d_out_values, int local_size, int grid_size) {
     (void)local_size;
     (void)grid_size;
-    // every reorder variant is the stable WG split with its LDS reorder buffer
+    // every reorder variant is an alias of the stable WG split with its LDS
+    // reorder buffer (see the header): same output, one schedule
     (void)reorder_type;
     return split_common(d_in, d_out, d_start, length, buckets, structure, d_in_values, d_out_values, EDPP_SPLIT_WG);
 }

@@ -13,8 +13,11 @@
 namespace edpp {
 
 // ---------------------------------------------------------------- look-back scan
+// `in` and `out` may be the same array (an in-place scan): neither pointer is
+// __restrict__, and every load of a tile precedes a barrier that precedes its
+// stores.
 template <class T, int BLOCK, int VPT>
-__global__ __launch_bounds__(BLOCK) void k_scan_lookback(const T* __restrict__ in, T* __restrict__ out,
+__global__ __launch_bounds__(BLOCK) void k_scan_lookback(const T* in, T* out,
                                                          int64_t n, typename LookbackFor<T>::type st,
                                                          uint32_t* tickets, T* d_total,
                                                          uint32_t spin_limit, int32_t* err, int64_t tiles) {
@@ -105,7 +108,7 @@ __global__ __launch_bounds__(1024) void k_scan_partials(T* p, int64_t m, T* d_to
 
 // RTS final pass: scan each tile from its offset
 template <class T, int BLOCK, int VPT>
-__global__ __launch_bounds__(BLOCK) void k_tile_scan_offset(const T* __restrict__ in, T* __restrict__ out, int64_t n,
+__global__ __launch_bounds__(BLOCK) void k_tile_scan_offset(const T* in, T* out, int64_t n,
                                                             const T* __restrict__ offsets) {
     typedef TileScan<T, BLOCK, VPT> TS;
     __shared__ typename TS::Smem sm;
@@ -118,7 +121,7 @@ __global__ __launch_bounds__(BLOCK) void k_tile_scan_offset(const T* __restrict_
 
 // SSA first pass: local exclusive scan of each tile, tile sums to partials
 template <class T, int BLOCK, int VPT>
-__global__ __launch_bounds__(BLOCK) void k_tile_scan_local(const T* __restrict__ in, T* __restrict__ out, int64_t n,
+__global__ __launch_bounds__(BLOCK) void k_tile_scan_local(const T* in, T* out, int64_t n,
                                                            T* __restrict__ partials) {
     typedef TileScan<T, BLOCK, VPT> TS;
     __shared__ typename TS::Smem sm;

@@ -37,7 +37,7 @@ struct TileScan {
     // vector lies below n; only the one vector that straddles n goes element
     // by element.  (A fully element-wise tail path holds 64-bit addresses for
     // every element at once and sizes the registers of the whole kernel.)
-    __device__ __forceinline__ void load(const T* __restrict__ in, int64_t base, int64_t n) {
+    __device__ __forceinline__ void load(const T* in, int64_t base, int64_t n) {
         const int t = threadIdx.x;
         if (base + TILE <= n) {
             #pragma unroll
@@ -134,7 +134,7 @@ struct TileScan {
         total = sm.total;
     }
 
-    __device__ __forceinline__ void store(T* __restrict__ out, int64_t base, int64_t n, T prefix) const {
+    __device__ __forceinline__ void store(T* out, int64_t base, int64_t n, T prefix) const {
         const int t = threadIdx.x;
         const bool full = base + TILE <= n;
         #pragma unroll

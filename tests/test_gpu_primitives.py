@@ -274,14 +274,14 @@ def test_gather_out_of_range_index_is_skipped(torch_gpu, ctx, orc):
 @pytest.mark.parametrize("kind", [0, 1])
 def test_gather_scatter_full_size_properties(torch_gpu, ctx, kind):
     # 2^28: gather of iota returns the index (test_gather.cpp:64-70), and
-    # scatter undoes gather, for several pass counts
+    # scatter undoes gather, for every pass count 1..16
     torch = torch_gpu
     n = 1 << 28
     idx = empty(torch, n, torch.int32)
     ctx.gen_perm(idx, 31, kind)
     iota = torch.arange(n, dtype=torch.int32, device="cuda")
     out = empty(torch, n, torch.int32)
-    for passes in (1, 4):
+    for passes in range(1, 17):
         ctx.gather(iota, out, idx, n, passes)
         assert torch.equal(out, idx)
         back = torch.zeros(n, dtype=torch.int32, device="cuda")
@@ -377,16 +377,17 @@ def test_split_unaligned_input(torch_gpu, ctx, orc, B):
     assert np.array_equal(host(kout), ko) and np.array_equal(host(vout), vo) and np.array_equal(host(st), start)
 
 
+@pytest.mark.parametrize("dist", [0, 1])
 @pytest.mark.parametrize("B", [2, 16, 256, 4096])
-def test_split_full_size_properties(torch_gpu, ctx, B):
-    # 2^28 pairs, Zipf keys, vals = original index: bucket ids non-decreasing,
-    # stable (vals increasing inside each bucket), a permutation, keys follow
-    # their values, offsets equal the bucket counts
+def test_split_full_size_properties(torch_gpu, ctx, B, dist):
+    # 2^28 pairs, uniform or Zipf keys, vals = original index: bucket ids
+    # non-decreasing, stable (vals increasing inside each bucket), a
+    # permutation, keys follow their values, offsets equal the bucket counts
     torch = torch_gpu
     n = 1 << 28
     keys = empty(torch, n, torch.int32)
     vals = empty(torch, n, torch.int32)
-    ctx.gen_split_keys(keys, vals, B, 1, 4242)
+    ctx.gen_split_keys(keys, vals, B, dist, 4242 + dist)
     kout = empty(torch, n, torch.int32)
     vout = empty(torch, n, torch.int32)
     st = empty(torch, B, torch.int32)
