@@ -46,6 +46,9 @@ def parse():
     p.add_argument("--train-calls", type=int, default=2000)
     p.add_argument("--test-calls", type=int, default=500)
     p.add_argument("--db-out", default=None, help="write the autotuner database (JSON lines) here")
+    p.add_argument("--headline-only", action="store_true",
+                   help="only the gather/scatter step: skip the per-configuration entries (scan, split, batched, "
+                        "learned) and the random-access probe")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-traffic", action="store_true",
                    help="skip the two rocprofv3 --pmc child runs that measure HBM bytes per launch")
@@ -158,8 +161,14 @@ def cpu_primitives(orc, ref, b, n, gsrc, gidx):
     reference's own when built) on n int32, and gather and scatter apart on
     the gather/scatter sample (gsrc, gidx).  Run 0 is first touch and
     discarded, then 5 timed runs; GB/s from the Hampel mean."""
-    src = orc.gen_i32(n, 0x5CA7) & 15
-    out = np.empty(n, np.int32)
+    def aligned(count):  # 32-byte aligned int32 array: the streaming-store kernels use aligned AVX2 moves
+        raw = np.empty(count + 8, np.int32)
+        off = (-raw.ctypes.data % 32) // 4
+        return raw[off:off + count]
+
+    src = aligned(n)
+    src[:] = orc.gen_i32(n, 0x5CA7) & 15
+    out = aligned(n)
     m = gsrc.size
     gout = np.empty(m, np.int32)
     A = orc._a
@@ -174,12 +183,18 @@ def cpu_primitives(orc, ref, b, n, gsrc, gidx):
         gather = lambda: b.baseline_gather(A(gsrc), A(gout), A(gidx), m)
         scatter = lambda: b.baseline_scatter(A(gsrc), A(gout), A(gidx), m)
     res = {}
-    for name, fn, nb, cnt, kind in [("scan_ssa", lambda: b.baseline_scan_ssa(A(src), A(out), n), 8, n, "port"),
-                                    ("scan_rts", lambda: b.baseline_scan_rts(A(src), A(out), n), 8, n, "port"),
-                                    ("copy", copy, 8, n, "reference" if ref is not None else "port"),
-                                    ("scale", scale, 8, n, "reference" if ref is not None else "port"),
-                                    ("gather", gather, 12, m, "reference" if ref is not None else "port"),
-                                    ("scatter", scatter, 12, m, "reference" if ref is not None else "port")]:
+    legs = [("scan_ssa", lambda: b.baseline_scan_ssa(A(src), A(out), n), 8, n, "port"),
+            ("scan_rts", lambda: b.baseline_scan_rts(A(src), A(out), n), 8, n, "port"),
+            ("copy", copy, 8, n, "reference" if ref is not None else "port"),
+            ("scale", scale, 8, n, "reference" if ref is not None else "port"),
+            ("gather", gather, 12, m, "reference" if ref is not None else "port"),
+            ("scatter", scatter, 12, m, "reference" if ref is not None else "port")]
+    if ref is not None:  # streaming-store copy and scale (openmp/test_bandwidth_CPU.cpp:58-83)
+        legs += [("copy_ss", lambda: ref.edpp_ref_copy_ss(A(src), A(out), n), 8, n, "reference"),
+                 ("scale_ss", lambda: ref.edpp_ref_scale_ss(A(src), A(out), n), 8, n, "reference")]
+    else:
+        res["copy_ss"] = res["scale_ss"] = "unavailable: the reference's OpenMP build (oracle/_ref) is absent"
+    for name, fn, nb, cnt, kind in legs:
         fn()
         ts = [fn() for _ in range(5)]
         res[name] = {"n": cnt, "gbs": round(gbs(nb * cnt, hampel(ts)), 3),
@@ -188,6 +203,8 @@ def cpu_primitives(orc, ref, b, n, gsrc, gidx):
             head = np.concatenate(([0], np.cumsum(src[:4095], dtype=np.int64))).astype(np.int32)
             tail = int(np.sum(src[:-1], dtype=np.int64) & 0xFFFFFFFF)
             res[name]["checked"] = bool(np.array_equal(out[:4096], head)) and int(out[-1]) & 0xFFFFFFFF == tail
+        elif name in ("copy_ss", "scale_ss"):  # whole output against the formula
+            res[name]["checked"] = bool(np.array_equal(out, src * (3 if name == "scale_ss" else 1)))
     if os.path.exists("/usr/include/tbb/task_scheduler_init.h"):
         res["tbb_scan"] = "not built: the TBB headers are present but this baseline does not link TBB"
     elif os.path.isdir("/usr/include/tbb") or os.path.isdir("/usr/include/oneapi/tbb"):
@@ -416,17 +433,35 @@ def main():
                           "gather_ms": round(g_ms, 4), "scatter_ms": round(s_ms, 4)},
     }
     del dst2
+    if not args.headline_only:
+        probe = random_probe(ctx, edpp, torch, src)
+        ceil = probe["table_1GiB"]["read"]["gather_equivalent_gbs"]
+        line["roofline"]["random_read_ceiling_gbs"] = ceil
+        line["roofline"]["frac_of_random_read_ceiling"] = round(achieved / ceil, 4)
+        line["random_access_probe"] = probe
+        line["configs"] = configs(ctx, edpp, torch, T, src, dst, idx, copy_gbs, line)
     if args.suite:
         line["suite"] = suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs)
-    if args.batched or args.learned or args.access:
+    big = args.batched or args.learned or args.access or not args.headline_only
+    if big:
         del src, dst, idx
         torch.cuda.empty_cache()
     if args.access:
         line["access"] = access(ctx, edpp, torch, T)
-    if args.batched:
-        line["batched"] = batched(ctx, edpp, torch)
-    if args.learned:
-        line["learned"] = learned(ctx, edpp, torch, args)
+    if args.batched or not args.headline_only:
+        bt = batched(ctx, edpp, torch)
+        if args.batched:
+            line["batched"] = bt
+        if not args.headline_only:
+            bt["frac_peak"] = round(bt["batched_gbs"] / HBM_PEAK_GBS, 4)
+            bt["frac_copy"] = round(bt["batched_gbs"] / copy_gbs, 4)
+            line["configs"]["C4_batched_10k_mix"] = bt
+    if args.learned or not args.headline_only:
+        ln = learned(ctx, edpp, torch, args)
+        if args.learned:
+            line["learned"] = ln
+        if not args.headline_only:
+            line["configs"]["C5_learned_schedule"] = ln
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args)
     s = json.dumps(line)
@@ -435,6 +470,105 @@ def main():
         with open(args.out, "w") as f:
             f.write(s + "\n")
     ctx.close()
+
+
+def random_probe(ctx, edpp, torch, table):
+    """Random 4-byte accesses with positions hashed in registers (no index
+    stream): 2^28 reads, then 2^28 writes, over tables of 1 GiB (HBM), 128 MiB
+    (Infinity Cache) and 2 MiB (each XCD's L2).  `gather_equivalent_gbs` is
+    12 B x accesses / time: the rate of a 12 B/element gather (or scatter)
+    whose only cost were its random accesses, the ceiling k_gather is graded
+    against."""
+    n = N_FULL
+    tot = torch.zeros(1, dtype=torch.int64, device="cuda")
+    scratch = table.clone()  # the probe's writes go to a copy of the step's source
+    res = {}
+    for lg, name in ((28, "table_1GiB"), (25, "table_128MiB"), (19, "table_2MiB")):
+        row = {}
+        for write in (False, True):
+            tab = scratch if write else table
+            ms = []
+            ctx.access_random(tab, n, 0xA11CE, write, tot, table_n=1 << lg)
+            for _ in range(7):
+                ms.append(ctx.access_random(tab, n, 0xA11CE, write, tot, table_n=1 << lg, timed=True))
+            t = hampel(ms)
+            row["write" if write else "read"] = {"ms": round(t, 4), "g_accesses_s": round(n / (t * 1e-3) / 1e9, 2),
+                                                 "gb_s_4byte": round(gbs(4.0 * n, t), 1),
+                                                 "gb_s_128byte_lines": round(gbs(128.0 * n, t), 1),
+                                                 "gather_equivalent_gbs": round(gbs(12.0 * n, t), 1)}
+        res[name] = row
+        print("random probe", name, row, file=sys.stderr, flush=True)
+    del scratch
+    return res
+
+
+def configs(ctx, edpp, torch, T, src, dst, idx, copy_gbs, line):
+    """One entry per benchmark configuration that fits this process's buffers:
+    C1 (exclusive scan of 2^24 values in [0,15], warm and cold, and 2^28), C2
+    (gather / scatter at 2^28: the default step, every pass count's best, and
+    the 4096-block-local permutation) and C3 (stable key/value split of 2^28
+    pairs, B in {2,16,256,4096}, uniform and Zipf(1) keys), each in GB/s of
+    algorithmic bytes, as a fraction of the 8 TB/s peak and of the measured
+    STREAM copy.  C4 and C5 are added by the caller."""
+    n = N_FULL
+    i32 = torch.int32
+    out = {"stream_copy_gbs": round(copy_gbs, 2)}
+
+    def entry(nbytes, t, cold=None):
+        g = gbs(nbytes, t["hampel_ms"])
+        e = {"gbs": round(g, 2), "ms": round(t["hampel_ms"], 4), "median_ms": round(t["median_ms"], 4),
+             "frac_peak": round(g / HBM_PEAK_GBS, 4), "frac_copy": round(g / copy_gbs, 4)}
+        if cold is not None:
+            gc = gbs(nbytes, cold["hampel_ms"])
+            e.update({"cold_gbs": round(gc, 2), "cold_ms": round(cold["hampel_ms"], 4),
+                      "cold_frac_peak": round(gc / HBM_PEAK_GBS, 4), "cold_frac_copy": round(gc / copy_gbs, 4)})
+        return e
+
+    b = torch.empty(n, dtype=i32, device="cuda")
+    # C1: look-back scan (the library's AUTO schedule); cold runs evict the
+    # 256 MiB Infinity Cache with a 1 GiB write before each run
+    scratch = torch.empty(1 << 28, dtype=i32, device="cuda")
+    flush = lambda: scratch.fill_(0x5A5A)
+    c1 = {}
+    for lg in (24, 28):
+        m = 1 << lg
+        x = b[:m]
+        ctx.gen_i32(x, 2024, 15)
+        t = T.time(lambda: ctx.scan(x, dst, m, edpp.SCAN_AUTO), reps=20)
+        tc = T.time(lambda: ctx.scan(x, dst, m, edpp.SCAN_AUTO), reps=20, flush=flush)
+        c1[f"2^{lg}"] = entry(8.0 * m, t, tc)
+    del scratch
+    out["C1_scan"] = c1
+    print("C1", c1, file=sys.stderr, flush=True)
+    # C2: the default step is the headline; add pass counts and block-local
+    c2 = {"default_step": {"gbs": line["value"], "gather_gbs": line["per_primitive"]["gather_gbs"],
+                           "scatter_gbs": line["per_primitive"]["scatter_gbs"],
+                           "frac_copy": round(line["value"] / copy_gbs, 4)}}
+    for kind, kname in ((edpp.PERM_RANDOM, "random"), (edpp.PERM_BLOCK_LOCAL, "block4096")):
+        ctx.gen_perm(idx, 0x5EED, kind)
+        for p in (1, 2, 4, 8, 16):
+            tg = T.time(lambda: ctx.gather(src, dst, idx, n, p), reps=5, warm=1)
+            ts = T.time(lambda: ctx.scatter(src, dst, idx, n, p), reps=5, warm=1)
+            c2[f"{kname}_p{p}"] = {"gather": entry(12.0 * n, tg), "scatter": entry(12.0 * n, ts)}
+    ctx.gen_perm(idx, 0x5EED, edpp.PERM_RANDOM)
+    out["C2_gather_scatter"] = c2
+    print("C2", {k: (v.get("gather", {}).get("gbs"), v.get("scatter", {}).get("gbs")) for k, v in c2.items()},
+          file=sys.stderr, flush=True)
+    # C3: stable split with bucket offsets, AUTO schedule
+    ko = torch.empty(n, dtype=i32, device="cuda")
+    vo = torch.empty(n, dtype=i32, device="cuda")
+    start = torch.empty(4096, dtype=i32, device="cuda")
+    c3 = {}
+    for B in (2, 16, 256, 4096):
+        for dist, dname in ((0, "uniform"), (1, "zipf")):
+            ctx.gen_split_keys(src, b, B, dist, 0x5917 + B)
+            t = T.time(lambda: ctx.split(src, ko, B, b, vo, start, n, edpp.SPLIT_AUTO), reps=20)
+            c3[f"B{B}_{dname}"] = entry(16.0 * n, t)
+    out["C3_split"] = c3
+    print("C3", {k: v["frac_copy"] for k, v in c3.items()}, file=sys.stderr, flush=True)
+    del ko, vo, b
+    ctx.gen_i32(src, 0xDA7A)
+    return out
 
 
 def suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs):
@@ -659,6 +793,13 @@ def batched(ctx, edpp, torch):
     t_run = med(lambda: b.run(timed=True), 5)
     t_graph = med(lambda: b.run(timed=True, graph=True), 5)
     t_plain = med(lambda: plain.run_plain(timed=True), 3)
+    # the batch's own autotuner: every tile shape x occupancy, median of 3
+    tune_t0 = time.time()
+    tune = b.autotune(reps=3)
+    tune_s = time.time() - tune_t0
+    shape, occ = b.get_shape()
+    t_tuned = med(lambda: b.run(timed=True, graph=True), 5)
+    fixed = (0, 4)  # the default shape and occupancy the launch uses without tuning
     # cold: a 1 GiB write evicts the 256 MiB Infinity Cache before each replay
     scratch = torch.empty(1 << 28, dtype=torch.int32, device="cuda")
 
@@ -681,11 +822,17 @@ def batched(ctx, edpp, torch):
     for p in plan:
         w = p["width"]
         nbytes += p["n"] * (2 * w if p["kind"] == 0 else (2 * (w + 4) if p["kind"] == 1 else 4 + 2 * w))
-    best = min(t_run, t_graph)
+    best = min(t_tuned, t_graph)
     out = {"jobs": len(plan), "elements": elems, "length_scale": scale, "tiles": b.info()["tiles"],
            "batched_ms": round(t_run, 4), "graph_ms": round(t_graph, 4), "graph_cold_ms": round(t_graph_cold, 4),
+           "autotuned_graph_ms": round(t_tuned, 4),
+           "autotuned": {"shape": shape, "wgs_per_cu": occ, "tile_shape": edpp.batch_shapes()[shape],
+                         "search_s": round(tune_s, 2),
+                         "fixed_default_ms": round(tune.get(fixed, -1.0), 4),
+                         "all_ms": {f"{k[0]}x{k[1]}": round(v, 4) for k, v in tune.items()}},
            "plain_per_call_ms": round(t_plain, 3),
-           "speedup": round(t_plain / best, 2), "batched_gbs": round(nbytes / (best * 1e-3) / 1e9, 1),
+           "speedup": round(t_plain / best, 2), "speedup_fixed_default_shape": round(t_plain / t_graph, 2),
+           "batched_gbs": round(nbytes / (best * 1e-3) / 1e9, 1),
            "jobs_differing_from_per_call": bad}
     print("batched", out, file=sys.stderr, flush=True)
     b.close()

@@ -50,6 +50,7 @@ _SIGS = {
     "edpp_hash_join_count": [c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_dp],
     "edpp_access_scale_i32": [c_vp, c_int, c_vp, c_vp, c_i64, c_int, c_dp],
     "edpp_access_chunks_i32": [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_dp],
+    "edpp_access_random_i32": [c_vp, c_vp, c_i64, c_i64, c_u64, c_int, c_vp, c_dp],
     "edpp_sort_u32": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_dp],
     "edpp_sort_i32": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_dp],
     "edpp_scan_exclusive_i32": [c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_dp],
@@ -94,6 +95,12 @@ _SIGS["edpp_batch_destroy"] = [c_vp]
 _SIGS["edpp_batch_set_occupancy"] = [c_vp, c_i32]
 _SIGS["edpp_batch_info"] = [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)]
 _SIGS["edpp_batch_run_plain"] = [c_vp, ctypes.POINTER(Job), c_i32, c_dp]
+_SIGS["edpp_batch_num_shapes"] = []
+_SIGS["edpp_batch_shape_info"] = [c_i32] + [ctypes.POINTER(c_i32)] * 5
+_SIGS["edpp_batch_set_shape"] = [c_vp, c_i32]
+_SIGS["edpp_batch_get_shape"] = [c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32)]
+_SIGS["edpp_batch_autotune"] = [c_vp, c_i32, ctypes.POINTER(ctypes.c_double)]
+BATCH_OCCUPANCIES = (2, 4, 8)
 
 OP_SCAN, OP_GATHER, OP_SCATTER, OP_SPLIT = 0, 1, 2, 3
 NUM_FEATURES, HIDDEN, MAX_CANDIDATES = 12, 64, 15
@@ -346,6 +353,13 @@ class Context:
         return self._call("edpp_access_chunks_i32", _buf(src, n, 4, "access src"), n, int(chunk),
                           _buf(order, chunks, 4, "access order"), _buf(count, 1, 8, "access count"), timed=timed)
 
+    def access_random(self, table, accesses, seed, write=False, total=None, table_n=None, timed=False):
+        """Random-access probe: `accesses` reads (total[0] = their sum) or
+        writes of table[fmix32(i + seed) & (table_n - 1)]."""
+        table_n = _count(_numel(table) if table_n is None else table_n, "access_random")
+        return self._call("edpp_access_random_i32", _buf(table, table_n, 4, "access table"), table_n, int(accesses),
+                          seed & (2**64 - 1), int(bool(write)), _buf(total, 1, 8, "access total"), timed=timed)
+
     def join_count(self, r_keys, s_keys, count, r_n=None, s_n=None, timed=False):
         """count[0] (uint64/int64 device tensor) = number of equal-key pairs of R and S."""
         r_n = _count(_numel(r_keys) if r_n is None else r_n, "join R")
@@ -434,6 +448,24 @@ class Batch:
     def set_occupancy(self, wgs_per_cu):
         self.ctx._check(self.ctx._L.edpp_batch_set_occupancy(self.h, int(wgs_per_cu)))
 
+    def set_shape(self, shape):
+        self.ctx._check(self.ctx._L.edpp_batch_set_shape(self.h, int(shape)))
+
+    def get_shape(self):
+        """(shape index, workgroups per compute unit) currently set."""
+        s, o = c_i32(), c_i32()
+        self.ctx._check(self.ctx._L.edpp_batch_get_shape(self.h, ctypes.byref(s), ctypes.byref(o)))
+        return s.value, o.value
+
+    def autotune(self, reps=3):
+        """Times every shape x occupancy (median of reps); leaves the fastest
+        set and returns {(shape, wgs_per_cu): ms}."""
+        k = self.ctx._L.edpp_batch_num_shapes()
+        out = (ctypes.c_double * (k * len(BATCH_OCCUPANCIES)))()
+        self.ctx._check(self.ctx._L.edpp_batch_autotune(self.h, int(reps), out))
+        return {(s, o): out[s * len(BATCH_OCCUPANCIES) + i] for s in range(k)
+                for i, o in enumerate(BATCH_OCCUPANCIES)}
+
     def info(self):
         a, b, c = c_i64(), c_i64(), c_i64()
         self.ctx._check(self.ctx._L.edpp_batch_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
@@ -455,6 +487,18 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def batch_shapes():
+    """[{block, scan_tile32, scan_tile64, split_tile, gather_tile}] of every batch tile shape."""
+    L = lib()
+    out = []
+    for k in range(L.edpp_batch_num_shapes()):
+        v = [c_i32() for _ in range(5)]
+        L.edpp_batch_shape_info(k, *[ctypes.byref(x) for x in v])
+        out.append(dict(zip(("block", "scan_tile32", "scan_tile64", "split_tile", "gather_tile"),
+                            (x.value for x in v))))
+    return out
 
 
 def make_call(op, n, src, dst, elem_bytes=4, idx=None, buckets=0, vals_in=None, vals_out=None):

@@ -77,6 +77,53 @@ __global__ void k_access_reset_ticket(uint32_t* ticket) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *ticket = 0u;
 }
 
+// MurmurHash3's 32-bit finaliser (Appleby): a bijection of 32-bit words
+// whose output bits all depend on every input bit.
+__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+
+// Random-access probe: access i touches table[fmix32(i + seed) & mask].
+// Workgroup g takes accesses [g*S, g*S + S) of each round of gridDim.x*S
+// (S = 256*U), lane t the accesses g*S + t + u*256, u < U: every lane keeps U
+// independent accesses in flight and the positions come from registers only.
+// Reads add the values up (64-bit, per wave, then one atomic); writes store
+// the position itself, so colliding accesses write the same word.
+constexpr int kRandomBlock = 256;
+template <int U, bool WRITE>
+__global__ __launch_bounds__(kRandomBlock) void k_access_random(int32_t* __restrict__ table, uint32_t mask,
+                                                                int64_t accesses, uint32_t seed,
+                                                                unsigned long long* __restrict__ sum) {
+    constexpr int64_t S = (int64_t)kRandomBlock * U;
+    const int64_t stride = (int64_t)gridDim.x * S;
+    uint64_t acc = 0;
+    for (int64_t base = (int64_t)blockIdx.x * S + threadIdx.x; base < accesses; base += stride) {
+        uint32_t p[U];
+        #pragma unroll
+        for (int u = 0; u < U; ++u) p[u] = fmix32((uint32_t)(base + (int64_t)u * kRandomBlock) + seed) & mask;
+        if (WRITE) {
+            #pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (base + (int64_t)u * kRandomBlock < accesses) table[p[u]] = (int32_t)p[u];
+        } else {
+            uint32_t v[U];
+            #pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = base + (int64_t)u * kRandomBlock < accesses ? (uint32_t)table[p[u]] : 0u;
+            #pragma unroll
+            for (int u = 0; u < U; ++u) acc += v[u];
+        }
+    }
+    if (!WRITE) {
+        const uint64_t w = wave_sum(acc);
+        if (lane_id() == 0 && w) atomicAdd(sum, (unsigned long long)w);
+    }
+}
+
 }  // namespace
 
 int launch_access_scale(edpp_ctx* c, int pattern, const int32_t* in, int32_t* out, int64_t n, int repeat) {
@@ -112,6 +159,29 @@ int launch_access_chunks(edpp_ctx* c, const int32_t* in, int64_t n, int64_t chun
     if (chunks > 0)
         hipLaunchKernelGGL(k_access_chunks, dim3((unsigned)chunks), dim3(256), 0, c->stream, in, n, chunk, order,
                            chunks, ticket, d_count);
+    EDPP_HIP(c, hipGetLastError());
+    return EDPP_OK;
+}
+
+}  // namespace edpp
+
+namespace edpp {
+
+int launch_access_random(edpp_ctx* c, int32_t* table, int64_t table_n, int64_t accesses, uint32_t seed, bool write,
+                         unsigned long long* d_sum) {
+    constexpr int U = 8;
+    if (!write) hipLaunchKernelGGL(k_access_zero, dim3(1), dim3(64), 0, c->stream, d_sum);
+    if (accesses > 0) {
+        const uint32_t mask = (uint32_t)(table_n - 1);
+        const int64_t groups = ceil_div(accesses, (int64_t)kRandomBlock * U);
+        const unsigned grid = (unsigned)std::min<int64_t>(groups, (int64_t)c->cus * 16);
+        if (write)
+            hipLaunchKernelGGL((k_access_random<U, true>), dim3(grid), dim3(kRandomBlock), 0, c->stream, table, mask,
+                               accesses, seed, d_sum);
+        else
+            hipLaunchKernelGGL((k_access_random<U, false>), dim3(grid), dim3(kRandomBlock), 0, c->stream, table, mask,
+                               accesses, seed, d_sum);
+    }
     EDPP_HIP(c, hipGetLastError());
     return EDPP_OK;
 }

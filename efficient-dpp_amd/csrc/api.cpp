@@ -538,6 +538,22 @@ int edpp_access_chunks_i32(edpp_ctx* c, const int32_t* in, int64_t n, int64_t ch
     return rc;
 }
 
+int edpp_access_random_i32(edpp_ctx* c, int32_t* table, int64_t table_n, int64_t accesses, uint64_t seed, int write,
+                           int64_t* d_sum, double* ms) {
+    EDPP_CHECK_CTX(c);
+    if (table_n < 1 || table_n > (1ll << 31) || (table_n & (table_n - 1)))
+        return fail(c, EDPP_EINVAL, "table_n=%lld must be a power of two in [1, 2^31]", (long long)table_n);
+    if (accesses < 0 || accesses >= (1ll << 32))
+        return fail(c, EDPP_EINVAL, "accesses=%lld outside [0, 2^32)", (long long)accesses);
+    if (!table || (!write && !d_sum)) return fail(c, EDPP_EINVAL, "access random: null buffer");
+    int rc = timing_begin(c, ms);
+    if (!rc)
+        rc = launch_access_random(c, table, table_n, accesses, (uint32_t)seed, write != 0,
+                                  (unsigned long long*)d_sum);
+    if (!rc) rc = timing_end(c, ms);
+    return rc;
+}
+
 // ---------------------------------------------------------------- hash join
 int edpp_hash_join_count(edpp_ctx* c, const int32_t* r_keys, int64_t r_n, const int32_t* s_keys, int64_t s_n,
                          uint64_t* d_count, double* ms) {

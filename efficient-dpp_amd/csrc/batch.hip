@@ -22,8 +22,6 @@ namespace edpp {
 
 namespace {
 
-constexpr int kBatchBlock = 256;
-static_assert(kBatchBlock == kSplitBlock, "split tiles assume the split block size");
 constexpr int kPhaseHist = 0, kPhaseHistScan = 1, kPhaseMain = 2, kPhaseScatter = 3;
 
 struct DJob {
@@ -36,8 +34,7 @@ struct DJob {
     int32_t* vout;
     void* aux;
     uint64_t* status;  // u32 scan: 1 word per tile; u64 scan: 2 per tile; split: 1 per histogram-scan tile
-    int32_t* his;      // split: [B][tiles], scanned in place (accessed with agent-scope atomics only)
-    uint32_t* ctr;     // split: [0] histogram tiles done, [1] histogram-scan tiles done
+    uint64_t* his;     // split: [B][tiles] tagged words {tag:32 | value:32}
 };
 
 struct Seg {
@@ -45,65 +42,69 @@ struct Seg {
     int32_t job, phase;
 };
 
-// keys per lane of a split tile: 8 keeps the persistent kernel at about 100
-// registers (16 for 32-bit keys needed 200, two workgroups per CU)
-constexpr int kBatchSplitIpt = 8;
+// Tile shapes of the persistent launch: lanes per workgroup, 16-byte vectors
+// per lane of a 32-bit and of a 64-bit scan tile, keys per lane of a split
+// tile, 16-byte index vectors per lane of a gather tile.  The autotuner
+// (edpp_batch_autotune) times every shape at several occupancies.
+struct BatchShape {
+    int block, scan_vpt32, scan_vpt64, split_ipt, gather_u;
+};
+constexpr BatchShape kShapes[] = {
+    {256, 8, 4, 8, 4},   // 0: scan 8192 / 2048, split 2048, gather 4096 (the default)
+    {256, 4, 2, 4, 2},   // 1: half-size tiles: more tiles for small jobs
+    {256, 8, 4, 8, 8},   // 2: gather tiles of 8192
+    {512, 4, 2, 8, 4},   // 3: 8 waves, scan 8192 / 2048, split 4096, gather 8192
+    {512, 8, 4, 4, 2},   // 4: 8 waves, scan 16384 / 4096, split 2048, gather 4096
+    {256, 16, 8, 8, 8},  // 5: scan 16384 / 4096, gather 8192
+};
+constexpr int kNumShapes = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
+constexpr int kDefaultShape = 0;
+constexpr int kOccupancies[] = {2, 4, 8};
+constexpr int kNumOcc = 3;
 
-// tile lengths per kind and width
-__host__ __device__ inline int64_t tile_elems(int kind, int wide) {
-    if (kind == EDPP_JOB_SCAN) return wide ? 2048 : 8192;  // TileScan<u64,256,4>, TileScan<u32,256,8>
-    if (kind == EDPP_JOB_SPLIT) return 256 * kBatchSplitIpt;  // 256 lanes x kBatchSplitIpt keys
-    return 4096;                                           // gather: 256 lanes x 16
+__host__ __device__ inline int64_t tile_elems(const BatchShape& s, int kind, int wide) {
+    if (kind == EDPP_JOB_SCAN) return (int64_t)s.block * (wide ? 2 * s.scan_vpt64 : 4 * s.scan_vpt32);
+    if (kind == EDPP_JOB_SPLIT) return (int64_t)s.block * s.split_ipt;
+    return (int64_t)s.block * 4 * s.gather_u;
 }
-constexpr int64_t kHistScanTile = 4096;  // TileScan<uint32_t,256,4>
+inline int64_t hscan_tile_elems(const BatchShape& s) { return (int64_t)s.block * 16; }  // TileScan<u32,BLOCK,4>
 
-// Cross-workgroup data of one launch (histograms, counters) moves through
-// agent-scope atomic loads and stores, which stay coherent across the XCDs'
-// L2s.  A finished tile is counted with a release increment and a waiter
-// that has seen its count takes one acquire fence, so the counts written
-// before the increment are ordered before the waiter's reads of them by the
-// memory model, not only by the hardware.  Polling stays relaxed: no spin
-// iteration invalidates a cache, only the one successful wait does.
+// Cross-workgroup data of one launch moves only in self-describing words:
+// look-back status words {flag | value} for the scans, and for the splits
+// histogram words {tag | value} whose tag says what the value is (a tile's
+// bucket count, then its scanned offset).  A reader polls each word it needs
+// with relaxed agent-scope atomic loads until the word shows the tag it
+// waits for, so it never infers one location's contents from another's and
+// needs no fence; the words are zeroed by the init kernel of every run.
+enum : uint32_t { kTagCount = 1, kTagScanned = 2 };
 
-// thread 0 waits for *p >= target, then acquires (agent scope) what the
-// release increments up to target published; the barrier passes that on to
-// the workgroup
-__device__ bool wait_count(uint32_t* p, uint32_t target, uint32_t spin_limit, int32_t* err, uint32_t* s_ok) {
-    if (threadIdx.x == 0) {
-        uint32_t spins = 0, ok = 1;
-        while (ld_relaxed(p) < target) {
-            if (++spins > spin_limit) {
-                st_relaxed(err, (int32_t)EDPP_ETIMEOUT);
-                ok = 0;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
+__device__ __forceinline__ uint64_t tagged(uint32_t tag, uint32_t v) { return ((uint64_t)tag << 32) | v; }
+
+// polls *p until its tag is `tag`; on watchdog expiry sets *err and returns
+// what it read (the run then reports EDPP_ETIMEOUT; consumers stay in bounds)
+__device__ __forceinline__ uint32_t wait_tagged(const uint64_t* p, uint32_t tag, uint32_t spin_limit, int32_t* err) {
+    uint64_t w = ld_relaxed(p);
+    uint32_t spins = 0;
+    while ((uint32_t)(w >> 32) != tag) {
+        if (++spins > spin_limit) {
+            st_relaxed(err, (int32_t)EDPP_ETIMEOUT);
+            break;
         }
-        if (ok) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        *s_ok = ok;
+        __builtin_amdgcn_s_sleep(1);
+        w = ld_relaxed(p);
     }
-    __syncthreads();
-    return *s_ok != 0;
-}
-
-// every lane's stores of this tile complete and reach thread 0 through the
-// barrier; thread 0 then counts the tile with a release increment
-__device__ void signal_count(uint32_t* p) {
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    return (uint32_t)w;
 }
 
 // ---------------------------------------------------------------- scan tiles
-template <class T, int VPT, bool COHERENT>
+template <class T, int BLOCK, int VPT>
 __device__ void scan_tile(const T* in, T* out, int64_t n, int64_t t, typename LookbackFor<T>::type st, T* total,
                           char* smem, uint32_t spin_limit, int32_t* err) {
-    typedef TileScan<T, kBatchBlock, VPT> TS;
+    typedef TileScan<T, BLOCK, VPT> TS;
     typename TS::Smem& sm = *reinterpret_cast<typename TS::Smem*>(smem);
     const int64_t base = t * TS::TILE;
     TS ts;
-    if (COHERENT) ts.load_coherent(in, base, n);
-    else ts.load(in, base, n);
+    ts.load(in, base, n);
     ts.scan(sm);
     if (threadIdx.x < kWave) {
         T prefix = 0;
@@ -120,51 +121,97 @@ __device__ void scan_tile(const T* in, T* out, int64_t n, int64_t t, typename Lo
         }
     }
     __syncthreads();
-    if (COHERENT) ts.store_coherent(out, base, n, sm.prefix);
-    else ts.store(out, base, n, sm.prefix);
+    ts.store(out, base, n, sm.prefix);
+}
+
+// In-place scan of a split job's histogram words: tile t of the bucket-major
+// matrix waits for the count words of its entries, scans them with the
+// look-back over the job's histogram-scan tiles, and rewrites them as
+// scanned words.  Only this tile reads or writes its entries' count words.
+template <int BLOCK>
+__device__ void hist_scan_tile(const DJob& J, int64_t t, char* smem, uint32_t spin_limit, int32_t* err) {
+    typedef TileScan<uint32_t, BLOCK, 4> TS;
+    typename TS::Smem& sm = *reinterpret_cast<typename TS::Smem*>(smem);
+    const int64_t hlen = ((int64_t)1 << J.lg) * J.tiles;
+    const int64_t base = t * TS::TILE;
+    TS ts;
+    #pragma unroll
+    for (int j = 0; j < 4; ++j)
+        #pragma unroll
+        for (int k = 0; k < TS::EPV; ++k) {
+            const int64_t e = base + (int64_t)j * TS::SUB + (int64_t)threadIdx.x * TS::EPV + k;
+            ts.v[j][k] = e < hlen ? wait_tagged(J.his + e, kTagCount, spin_limit, err) : 0u;
+        }
+    ts.scan(sm);
+    LookbackState32 st;
+    st.status = J.status;
+    st.epoch = 1;
+    if (threadIdx.x < kWave) {
+        uint32_t prefix = 0;
+        if (t == 0) {
+            if (threadIdx.x == 0) st.publish(0, kFlagPrefix, ts.total);
+        } else {
+            if (threadIdx.x == 0) st.publish(t, kFlagAggregate, ts.total);
+            prefix = lookback<uint32_t>(st, t, spin_limit, err);
+            if (threadIdx.x == 0) st.publish(t, kFlagPrefix, prefix + ts.total);
+        }
+        if (threadIdx.x == 0) sm.prefix = prefix;
+    }
+    __syncthreads();
+    #pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        uint32_t x = sm.prefix + ts.off[j];
+        #pragma unroll
+        for (int k = 0; k < TS::EPV; ++k) {
+            const int64_t e = base + (int64_t)j * TS::SUB + (int64_t)threadIdx.x * TS::EPV + k;
+            if (e < hlen) st_relaxed(J.his + e, tagged(kTagScanned, x));
+            x += ts.v[j][k];
+        }
+    }
 }
 
 // ---------------------------------------------------------------- split tiles
-template <class K, int IPT>
+template <class K, int IPT, int BLOCK>
 __device__ void split_hist_tile(const DJob& J, int64_t t, uint32_t* lds) {
+    constexpr int W = BLOCK / kWave;
     const int B = 1 << J.lg;
     const int w = threadIdx.x / kWave;
-    constexpr int S = kBatchBlock * IPT;
+    constexpr int S = BLOCK * IPT;
     uint32_t* whist = lds;
-    for (int b = threadIdx.x; b < kSplitWaves * B; b += kBatchBlock) whist[b] = 0;
+    for (int b = threadIdx.x; b < W * B; b += BLOCK) whist[b] = 0;
     __syncthreads();
     const int64_t base = t * S, end = std::min<int64_t>(base + S, J.n);
     if (J.lg <= kBallotMatchBits) count_subtile<K, IPT, true>((const K*)J.in, base, end, J.lg, 0, whist + w * B);
     else count_subtile<K, IPT, false>((const K*)J.in, base, end, J.lg, 0, whist + w * B);
     __syncthreads();
-    for (int b = threadIdx.x; b < B; b += kBatchBlock) {
+    for (int b = threadIdx.x; b < B; b += BLOCK) {
         uint32_t a = 0;
-        for (int q = 0; q < kSplitWaves; ++q) a += whist[q * B + b];
-        st_relaxed(J.his + (int64_t)b * J.tiles + t, (int32_t)a);
+        for (int q = 0; q < W; ++q) a += whist[q * B + b];
+        st_relaxed(J.his + (int64_t)b * J.tiles + t, tagged(kTagCount, a));
     }
-    signal_count(J.ctr + 0);
 }
 
-template <class K, int IPT>
-__device__ void split_scatter_tile(const DJob& J, int64_t t, uint32_t* lds) {
+template <class K, int IPT, int BLOCK>
+__device__ void split_scatter_tile(const DJob& J, int64_t t, uint32_t* lds, uint32_t spin_limit, int32_t* err) {
+    constexpr int W = BLOCK / kWave;
     const int B = 1 << J.lg;
-    constexpr int S = kBatchBlock * IPT;
+    constexpr int S = BLOCK * IPT;
     uint64_t* wmask = reinterpret_cast<uint64_t*>(lds);
-    uint32_t* whist = lds + whist_word_offset(B);
-    uint32_t* run = whist + kSplitWaves * B;
+    uint32_t* whist = lds + whist_word_offset(B, W);
+    uint32_t* run = whist + W * B;
     uint32_t* sub = run + B;
     uint32_t* off = sub + B;
     uint32_t* wtmp = off + B;
-    K* kbuf = reinterpret_cast<K*>(lds + kbuf_word_offset(B));
+    K* kbuf = reinterpret_cast<K*>(lds + kbuf_word_offset(B, W));
     int32_t* vbuf = reinterpret_cast<int32_t*>(kbuf + S);
     const int w = threadIdx.x / kWave;
     const bool kv = J.vin != nullptr;
-    for (int b = threadIdx.x; b < B; b += kBatchBlock) {
-        uint32_t r = (uint32_t)ld_relaxed(J.his + (int64_t)b * J.tiles + t);
+    for (int b = threadIdx.x; b < B; b += BLOCK) {
+        const uint32_t r = wait_tagged(J.his + (int64_t)b * J.tiles + t, kTagScanned, spin_limit, err);
         run[b] = r;
         if (t == 0 && J.aux) static_cast<int32_t*>(J.aux)[b] = (int32_t)r;
     }
-    for (int b = threadIdx.x; b < kSplitWaves * B; b += kBatchBlock) {
+    for (int b = threadIdx.x; b < W * B; b += BLOCK) {
         whist[b] = 0;
         wmask[b] = 0ull;
     }
@@ -182,23 +229,24 @@ __device__ void split_scatter_tile(const DJob& J, int64_t t, uint32_t* lds) {
     __syncthreads();
     const int cnt = (int)(tend - s);
     if (kv)
-        place_and_store<K, IPT, true>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk, key, val, B, 0, cnt,
-                                      static_cast<K*>(J.out), J.vout, J.n);
+        place_and_store<K, IPT, true, BLOCK>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk, key, val, B, 0, cnt,
+                                             static_cast<K*>(J.out), J.vout, J.n);
     else
-        place_and_store<K, IPT, false>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk, key, val, B, 0, cnt,
-                                       static_cast<K*>(J.out), nullptr, J.n);
+        place_and_store<K, IPT, false, BLOCK>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk, key, val, B, 0, cnt,
+                                              static_cast<K*>(J.out), nullptr, J.n);
 }
 
 // ---------------------------------------------------------------- gather tiles
-template <class T>
+// A tile of BLOCK*4*U elements: U rounds of one 16-byte index vector per lane
+template <class T, int BLOCK, int U>
 __device__ void gather_tile(const DJob& J, int64_t t) {
     const T* in = static_cast<const T*>(J.in);
     T* out = static_cast<T*>(J.out);
     const uint32_t n = (uint32_t)J.n;
-    const int64_t base = t * 4096;
+    const int64_t base = t * (int64_t)BLOCK * 4 * U;
     #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const int64_t e0 = base + (int64_t)u * 1024 + threadIdx.x * 4;
+    for (int u = 0; u < U; ++u) {
+        const int64_t e0 = base + (int64_t)u * BLOCK * 4 + threadIdx.x * 4;
         if (e0 + 4 <= J.n) {
             v4u32 x = ld_stream(reinterpret_cast<const v4u32*>(J.idx + e0));
             T v[4];
@@ -234,11 +282,12 @@ __global__ void k_batch_init(uint32_t* state, int64_t words, const DJob* jobs, i
     }
 }
 
-__global__ __launch_bounds__(kBatchBlock) void k_batch(const DJob* __restrict__ jobs, const Seg* __restrict__ segs,
-                                                       int32_t nseg, int64_t total, uint32_t* ticket,
-                                                       uint32_t spin_limit, int32_t* err) {
+template <int BLOCK, int SV32, int SV64, int IPT, int GU>
+__global__ __launch_bounds__(BLOCK) void k_batch(const DJob* __restrict__ jobs, const Seg* __restrict__ segs,
+                                                 int32_t nseg, int64_t total, uint32_t* ticket, uint32_t spin_limit,
+                                                 int32_t* err) {
     extern __shared__ __align__(16) char smem[];
-    __shared__ uint32_t s_ticket, s_ok;
+    __shared__ uint32_t s_ticket;
     while (true) {
         if (threadIdx.x == 0) s_ticket = fetch_add(ticket, 1u);
         __syncthreads();
@@ -256,48 +305,54 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const DJob* __restrict__ 
         const int64_t t = tk - sg.first;
         uint32_t* lds = reinterpret_cast<uint32_t*>(smem);
         if (sg.phase == kPhaseHist) {
-            if (J.wide) split_hist_tile<uint64_t, kBatchSplitIpt>(J, t, lds);
-            else split_hist_tile<uint32_t, kBatchSplitIpt>(J, t, lds);
+            if (J.wide) split_hist_tile<uint64_t, IPT, BLOCK>(J, t, lds);
+            else split_hist_tile<uint32_t, IPT, BLOCK>(J, t, lds);
         } else if (sg.phase == kPhaseHistScan) {
-            wait_count(J.ctr + 0, (uint32_t)J.tiles, spin_limit, err, &s_ok);
-            LookbackState32 st;
-            st.status = J.status;
-            st.epoch = 1;
-            const int64_t hlen = ((int64_t)1 << J.lg) * J.tiles;
-            scan_tile<uint32_t, 4, true>((const uint32_t*)J.his, (uint32_t*)J.his, hlen, t, st, nullptr, smem,
-                                         spin_limit, err);
-            signal_count(J.ctr + 1);
+            hist_scan_tile<BLOCK>(J, t, smem, spin_limit, err);
         } else if (sg.phase == kPhaseScatter) {
-            wait_count(J.ctr + 1, (uint32_t)J.hscan_tiles, spin_limit, err, &s_ok);
-            if (J.wide) split_scatter_tile<uint64_t, kBatchSplitIpt>(J, t, lds);
-            else split_scatter_tile<uint32_t, kBatchSplitIpt>(J, t, lds);
+            if (J.wide) split_scatter_tile<uint64_t, IPT, BLOCK>(J, t, lds, spin_limit, err);
+            else split_scatter_tile<uint32_t, IPT, BLOCK>(J, t, lds, spin_limit, err);
         } else if (J.kind == EDPP_JOB_SCAN) {
             if (J.wide) {
                 LookbackState64 st;
                 st.status = J.status;
                 st.epoch = 1;
-                scan_tile<uint64_t, 4, false>((const uint64_t*)J.in, (uint64_t*)J.out, J.n, t, st, (uint64_t*)J.aux,
-                                              smem, spin_limit, err);
+                scan_tile<uint64_t, BLOCK, SV64>((const uint64_t*)J.in, (uint64_t*)J.out, J.n, t, st, (uint64_t*)J.aux,
+                                                 smem, spin_limit, err);
             } else {
                 LookbackState32 st;
                 st.status = J.status;
                 st.epoch = 1;
-                scan_tile<uint32_t, 8, false>((const uint32_t*)J.in, (uint32_t*)J.out, J.n, t, st, (uint32_t*)J.aux,
-                                              smem, spin_limit, err);
+                scan_tile<uint32_t, BLOCK, SV32>((const uint32_t*)J.in, (uint32_t*)J.out, J.n, t, st, (uint32_t*)J.aux,
+                                                 smem, spin_limit, err);
             }
         } else {
-            if (J.wide) gather_tile<uint64_t>(J, t);
-            else gather_tile<uint32_t>(J, t);
+            if (J.wide) gather_tile<uint64_t, BLOCK, GU>(J, t);
+            else gather_tile<uint32_t, BLOCK, GU>(J, t);
         }
         __syncthreads();  // LDS of this tile is dead before the next one starts
     }
 }
 
-// LDS of the largest tile: the split scatter with 64-bit keys at 256 buckets
-// (ranking masks, counts and offsets, then 2048 keys and 2048 values): 40000
-// bytes, so four workgroups share a CU
-constexpr size_t kBatchLds =
-    (size_t)kbuf_word_offset(256) * 4 + (size_t)kBatchBlock * kBatchSplitIpt * (sizeof(uint64_t) + sizeof(int32_t));
+// LDS of the largest tile of a shape: the split scatter with 64-bit keys at
+// 256 buckets (ranking masks, counts and offsets, then the keys and values of
+// one tile), or a scan tile's scratch
+constexpr size_t shape_lds(const BatchShape& s) {
+    return std::max<size_t>((size_t)kbuf_word_offset(256, s.block / kWave) * 4 +
+                                (size_t)s.block * s.split_ipt * (sizeof(uint64_t) + sizeof(int32_t)),
+                            1024);
+}
+
+typedef void (*BatchKernel)(const DJob*, const Seg*, int32_t, int64_t, uint32_t*, uint32_t, int32_t*);
+
+template <int I>
+constexpr BatchKernel kernel_of() {
+    return &k_batch<kShapes[I].block, kShapes[I].scan_vpt32, kShapes[I].scan_vpt64, kShapes[I].split_ipt,
+                    kShapes[I].gather_u>;
+}
+const BatchKernel kKernels[kNumShapes] = {kernel_of<0>(), kernel_of<1>(), kernel_of<2>(),
+                                          kernel_of<3>(), kernel_of<4>(), kernel_of<5>()};
+static_assert(kNumShapes == 6, "one kernel per shape");
 
 }  // namespace
 }  // namespace edpp
@@ -307,18 +362,36 @@ using namespace edpp;
 struct edpp_batch {
     edpp_ctx* ctx = nullptr;
     std::vector<edpp_job> jobs;
+    int shape = kDefaultShape;
+    int wgs_per_cu = 4;
+    // the plan of the current shape
     DJob* d_jobs = nullptr;
     Seg* d_segs = nullptr;
     int32_t nseg = 0;
     int64_t total = 0, hist_tiles = 0, hscan_tiles = 0;
-    uint32_t* state = nullptr;  // zeroed each run: ticket, counters, look-back status and flags
+    uint32_t* state = nullptr;  // zeroed each run: ticket, look-back status, split histogram words
     int64_t state_words = 0;
-    void* rest = nullptr;       // split histograms (no reset needed)
-    int wgs_per_cu = 4;
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
+
+static void drop_graph(edpp_batch* b) {
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+static void drop_plan(edpp_batch* b) {
+    drop_graph(b);
+    hipFree(b->state);
+    hipFree(b->d_jobs);
+    hipFree(b->d_segs);
+    b->state = nullptr;
+    b->d_jobs = nullptr;
+    b->d_segs = nullptr;
+}
 
 static int batch_launch(edpp_batch* b, hipStream_t s) {
     edpp_ctx* c = b->ctx;
@@ -326,9 +399,11 @@ static int batch_launch(edpp_batch* b, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_init, dim3((unsigned)init_grid), dim3(256), 0, s, b->state, b->state_words,
                        (const DJob*)b->d_jobs, (int32_t)b->jobs.size());
     if (b->total > 0) {
+        const BatchShape& sh = kShapes[b->shape];
         const int64_t grid = std::min<int64_t>(b->total, (int64_t)c->cus * b->wgs_per_cu);
-        hipLaunchKernelGGL(k_batch, dim3((unsigned)grid), dim3(kBatchBlock), kBatchLds, s, (const DJob*)b->d_jobs,
-                           (const Seg*)b->d_segs, b->nseg, b->total, b->state, c->spin_limit, c->d_err);
+        hipLaunchKernelGGL(kKernels[b->shape], dim3((unsigned)grid), dim3(sh.block), shape_lds(sh), s,
+                           (const DJob*)b->d_jobs, (const Seg*)b->d_segs, b->nseg, b->total, b->state,
+                           c->spin_limit, c->d_err);
     }
     EDPP_HIP(c, hipGetLastError());
     return EDPP_OK;
@@ -336,18 +411,40 @@ static int batch_launch(edpp_batch* b, hipStream_t s) {
 
 static bool a16(const void* p) { return p == nullptr || aligned16(p); }
 
-extern "C" {
-
-int edpp_batch_create(edpp_ctx* c, const edpp_job* jobs, int32_t njobs, edpp_batch** out) {
-    if (!c || !out || njobs < 0 || (njobs > 0 && !jobs)) return c ? fail(c, EDPP_EINVAL, "batch: bad arguments") : EDPP_EINVAL;
-    *out = nullptr;
-    std::vector<DJob> dj(njobs);
-    // state layout in 32-bit words: [0] ticket, then per job its status/flags/counters
-    int64_t words = 64;  // ticket and padding
-    int64_t rest_bytes = 0;
-    std::vector<int64_t> soff(njobs, -1), roff(njobs, -1);
+// Validates the jobs (shape-independent); fills the shape-independent fields
+static int validate_jobs(edpp_ctx* c, const edpp_job* jobs, int32_t njobs) {
     for (int32_t j = 0; j < njobs; ++j) {
         const edpp_job& q = jobs[j];
+        if (q.kind < EDPP_JOB_SCAN || q.kind > EDPP_JOB_GATHER || (q.elem_bytes != 4 && q.elem_bytes != 8) ||
+            q.n < 0 || q.n >= (1ll << 31))
+            return fail(c, EDPP_EINVAL, "batch job %d: bad kind, width or length", j);
+        if (q.n > 0 && (!q.in || !q.out || (q.kind == EDPP_JOB_GATHER && !q.idx)))
+            return fail(c, EDPP_EINVAL, "batch job %d: missing buffer", j);
+        if (q.kind == EDPP_JOB_SPLIT) {
+            if (q.buckets < 1 || q.buckets > 256 || (q.buckets & (q.buckets - 1)))
+                return fail(c, EDPP_EINVAL, "batch job %d: buckets must be a power of two <= 256", j);
+            if ((q.vals_in == nullptr) != (q.vals_out == nullptr))
+                return fail(c, EDPP_EINVAL, "batch job %d: values need both vals_in and vals_out", j);
+        }
+        if (!a16(q.in) || !a16(q.out) || !a16(q.idx) || !a16(q.vals_in) || !a16(q.vals_out))
+            return fail(c, EDPP_EINVAL, "batch job %d: buffers must be 16-byte aligned", j);
+    }
+    return EDPP_OK;
+}
+
+// Builds the work queue and state of b->jobs for b->shape and uploads them.
+static int plan(edpp_batch* b) {
+    edpp_ctx* c = b->ctx;
+    const BatchShape& sh = kShapes[b->shape];
+    const int32_t njobs = (int32_t)b->jobs.size();
+    drop_plan(b);
+    std::vector<DJob> dj(njobs);
+    // state layout in 32-bit words: [0] ticket, then per job its look-back
+    // status words and, for a split, its tagged histogram words
+    int64_t words = 64;  // ticket and padding
+    std::vector<int64_t> soff(njobs, -1);
+    for (int32_t j = 0; j < njobs; ++j) {
+        const edpp_job& q = b->jobs[j];
         DJob& d = dj[j];
         d = DJob();
         d.kind = q.kind;
@@ -359,30 +456,16 @@ int edpp_batch_create(edpp_ctx* c, const edpp_job* jobs, int32_t njobs, edpp_bat
         d.vin = q.vals_in;
         d.vout = q.vals_out;
         d.aux = q.aux;
-        if (q.kind < EDPP_JOB_SCAN || q.kind > EDPP_JOB_GATHER || (q.elem_bytes != 4 && q.elem_bytes != 8) ||
-            q.n < 0 || q.n >= (1ll << 31))
-            return fail(c, EDPP_EINVAL, "batch job %d: bad kind, width or length", j);
-        if (q.n > 0 && (!q.in || !q.out || (q.kind == EDPP_JOB_GATHER && !q.idx)))
-            return fail(c, EDPP_EINVAL, "batch job %d: missing buffer", j);
-        if (q.kind == EDPP_JOB_SPLIT) {
-            if (q.buckets < 1 || q.buckets > 256 || (q.buckets & (q.buckets - 1)))
-                return fail(c, EDPP_EINVAL, "batch job %d: buckets must be a power of two <= 256", j);
-            if ((q.vals_in == nullptr) != (q.vals_out == nullptr))
-                return fail(c, EDPP_EINVAL, "batch job %d: values need both vals_in and vals_out", j);
+        if (q.kind == EDPP_JOB_SPLIT)
             while ((1 << d.lg) < q.buckets) ++d.lg;
-        }
-        if (!a16(q.in) || !a16(q.out) || !a16(q.idx) || !a16(q.vals_in) || !a16(q.vals_out))
-            return fail(c, EDPP_EINVAL, "batch job %d: buffers must be 16-byte aligned", j);
-        d.tiles = ceil_div(q.n, tile_elems(q.kind, d.wide));
+        d.tiles = ceil_div(q.n, tile_elems(sh, q.kind, d.wide));
         soff[j] = words;
         if (q.kind == EDPP_JOB_SCAN) {
             words += (d.wide ? 4 : 2) * d.tiles;  // one or two u64 status words per tile
         } else if (q.kind == EDPP_JOB_SPLIT) {
             const int64_t hlen = ((int64_t)1 << d.lg) * d.tiles;
-            d.hscan_tiles = ceil_div(hlen, kHistScanTile);
-            words += 2 + 2 * d.hscan_tiles;  // counters, then u64 status
-            roff[j] = rest_bytes;
-            rest_bytes += ((hlen * 4 + 255) / 256) * 256;
+            d.hscan_tiles = ceil_div(hlen, hscan_tile_elems(sh));
+            words += 2 * d.hscan_tiles + 2 * hlen;  // u64 status, then u64 histogram words
         }
         words = (words + 3) & ~(int64_t)3;  // 16-byte alignment of every job's state
     }
@@ -407,51 +490,48 @@ int edpp_batch_create(edpp_ctx* c, const edpp_job* jobs, int32_t njobs, edpp_bat
     for (int32_t j = 0; j < njobs; ++j)
         if (dj[j].kind == EDPP_JOB_SPLIT) add(j, kPhaseScatter, dj[j].tiles);
     if (tk >= (1ll << 32)) return fail(c, EDPP_EUNSUP, "batch of %lld tiles is too large", (long long)tk);
-
-    edpp_batch* b = new edpp_batch();
-    b->ctx = c;
-    b->jobs.assign(jobs, jobs + njobs);
     b->total = tk;
     b->hist_tiles = hist;
     b->hscan_tiles = hscan;
     b->nseg = (int32_t)segs.size();
     b->state_words = words;
-    int rc = EDPP_OK;
-    do {
-        if (hipMalloc(&b->state, (size_t)words * 4) != hipSuccess) { rc = EDPP_ENOMEM; break; }
-        if (rest_bytes && hipMalloc(&b->rest, (size_t)rest_bytes) != hipSuccess) { rc = EDPP_ENOMEM; break; }
-        for (int32_t j = 0; j < njobs; ++j) {
-            DJob& d = dj[j];
-            uint32_t* st = b->state + soff[j];
-            char* rs = roff[j] >= 0 ? static_cast<char*>(b->rest) + roff[j] : nullptr;
-            if (d.kind == EDPP_JOB_SCAN) {
-                d.status = reinterpret_cast<uint64_t*>(st);
-            } else if (d.kind == EDPP_JOB_SPLIT) {
-                d.ctr = st;
-                d.status = reinterpret_cast<uint64_t*>(st + 2);
-                d.his = reinterpret_cast<int32_t*>(rs);
-            }
+    if (hipMalloc(&b->state, (size_t)words * 4) != hipSuccess) return fail(c, EDPP_ENOMEM, "batch state");
+    for (int32_t j = 0; j < njobs; ++j) {
+        DJob& d = dj[j];
+        uint32_t* st = b->state + soff[j];
+        if (d.kind == EDPP_JOB_SCAN) {
+            d.status = reinterpret_cast<uint64_t*>(st);
+        } else if (d.kind == EDPP_JOB_SPLIT) {
+            d.status = reinterpret_cast<uint64_t*>(st);
+            d.his = reinterpret_cast<uint64_t*>(st + 2 * d.hscan_tiles);
         }
-        if (hipMalloc(&b->d_jobs, sizeof(DJob) * std::max<size_t>(1, dj.size())) != hipSuccess ||
-            hipMalloc(&b->d_segs, sizeof(Seg) * std::max<size_t>(1, segs.size())) != hipSuccess) {
-            rc = EDPP_ENOMEM;
-            break;
-        }
-        if ((njobs && hipMemcpy(b->d_jobs, dj.data(), sizeof(DJob) * dj.size(), hipMemcpyHostToDevice) != hipSuccess) ||
-            (!segs.empty() &&
-             hipMemcpy(b->d_segs, segs.data(), sizeof(Seg) * segs.size(), hipMemcpyHostToDevice) != hipSuccess)) {
-            rc = EDPP_EHIP;
-            break;
-        }
-        if (hipFuncSetAttribute((const void*)k_batch, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBatchLds) !=
-            hipSuccess) {
-            rc = EDPP_EHIP;
-            break;
-        }
-    } while (0);
+    }
+    if (hipMalloc(&b->d_jobs, sizeof(DJob) * std::max<size_t>(1, dj.size())) != hipSuccess ||
+        hipMalloc(&b->d_segs, sizeof(Seg) * std::max<size_t>(1, segs.size())) != hipSuccess)
+        return fail(c, EDPP_ENOMEM, "batch job table");
+    if ((njobs && hipMemcpy(b->d_jobs, dj.data(), sizeof(DJob) * dj.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+        (!segs.empty() &&
+         hipMemcpy(b->d_segs, segs.data(), sizeof(Seg) * segs.size(), hipMemcpyHostToDevice) != hipSuccess))
+        return fail(c, EDPP_EHIP, "batch: upload of the job table failed");
+    EDPP_HIP(c, hipFuncSetAttribute((const void*)kKernels[b->shape], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)shape_lds(sh)));
+    return EDPP_OK;
+}
+
+extern "C" {
+
+int edpp_batch_create(edpp_ctx* c, const edpp_job* jobs, int32_t njobs, edpp_batch** out) {
+    if (!c || !out || njobs < 0 || (njobs > 0 && !jobs)) return c ? fail(c, EDPP_EINVAL, "batch: bad arguments") : EDPP_EINVAL;
+    *out = nullptr;
+    int rc = validate_jobs(c, jobs, njobs);
+    if (rc) return rc;
+    edpp_batch* b = new edpp_batch();
+    b->ctx = c;
+    b->jobs.assign(jobs, jobs + njobs);
+    rc = plan(b);
     if (rc) {
         edpp_batch_destroy(b);
-        return fail(c, rc, "batch: device allocation or upload failed");
+        return rc;
     }
     *out = b;
     return EDPP_OK;
@@ -459,13 +539,39 @@ int edpp_batch_create(edpp_ctx* c, const edpp_job* jobs, int32_t njobs, edpp_bat
 
 int edpp_batch_set_occupancy(edpp_batch* b, int32_t wgs_per_cu) {
     if (!b || wgs_per_cu < 1 || wgs_per_cu > 8) return EDPP_EINVAL;
-    if (wgs_per_cu != b->wgs_per_cu && b->exec) {
-        hipGraphExecDestroy(b->exec);
-        hipGraphDestroy(b->graph);
-        b->exec = nullptr;
-        b->graph = nullptr;
-    }
+    if (wgs_per_cu != b->wgs_per_cu) drop_graph(b);
     b->wgs_per_cu = wgs_per_cu;
+    return EDPP_OK;
+}
+
+int edpp_batch_num_shapes(void) { return kNumShapes; }
+
+int edpp_batch_shape_info(int32_t shape, int32_t* block, int32_t* scan_tile32, int32_t* scan_tile64,
+                          int32_t* split_tile, int32_t* gather_tile) {
+    if (shape < 0 || shape >= kNumShapes) return EDPP_EINVAL;
+    const BatchShape& s = kShapes[shape];
+    if (block) *block = s.block;
+    if (scan_tile32) *scan_tile32 = (int32_t)tile_elems(s, EDPP_JOB_SCAN, 0);
+    if (scan_tile64) *scan_tile64 = (int32_t)tile_elems(s, EDPP_JOB_SCAN, 1);
+    if (split_tile) *split_tile = (int32_t)tile_elems(s, EDPP_JOB_SPLIT, 0);
+    if (gather_tile) *gather_tile = (int32_t)tile_elems(s, EDPP_JOB_GATHER, 0);
+    return EDPP_OK;
+}
+
+int edpp_batch_set_shape(edpp_batch* b, int32_t shape) {
+    if (!b) return EDPP_EINVAL;
+    if (shape < 0 || shape >= kNumShapes) return fail(b->ctx, EDPP_EINVAL, "batch shape %d outside [0, %d)", shape,
+                                                      kNumShapes);
+    if (shape == b->shape && b->state) return EDPP_OK;
+    EDPP_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));  // a run may still use the old plan
+    b->shape = shape;
+    return plan(b);
+}
+
+int edpp_batch_get_shape(edpp_batch* b, int32_t* shape, int32_t* wgs_per_cu) {
+    if (!b) return EDPP_EINVAL;
+    if (shape) *shape = b->shape;
+    if (wgs_per_cu) *wgs_per_cu = b->wgs_per_cu;
     return EDPP_OK;
 }
 
@@ -507,16 +613,50 @@ int edpp_batch_run_graph(edpp_batch* b, double* ms) {
     return timing_end(c, ms);
 }
 
+// Exhaustive autotune of the batch's launch: every tile shape at every
+// occupancy of kOccupancies, one warm-up run and `reps` timed runs each, the
+// median kept; the fastest pair is left set.  Each run recomputes every job's
+// output from its inputs, so no job may write its own input.
+int edpp_batch_autotune(edpp_batch* b, int32_t reps, double* times) {
+    if (!b || reps < 1) return b ? fail(b->ctx, EDPP_EINVAL, "batch autotune: reps < 1") : EDPP_EINVAL;
+    edpp_ctx* c = b->ctx;
+    for (size_t j = 0; j < b->jobs.size(); ++j) {
+        const edpp_job& q = b->jobs[j];
+        if (q.n > 0 && (q.in == q.out || (q.vals_in && q.vals_in == q.vals_out)))
+            return fail(c, EDPP_EINVAL, "batch autotune: job %zu works in place, repeated runs would change it", j);
+    }
+    double best = -1.0;
+    int best_shape = b->shape, best_occ = b->wgs_per_cu;
+    std::vector<double> ts((size_t)reps);
+    for (int s = 0; s < kNumShapes; ++s) {
+        int rc = edpp_batch_set_shape(b, s);
+        if (rc) return rc;
+        for (int o = 0; o < kNumOcc; ++o) {
+            edpp_batch_set_occupancy(b, kOccupancies[o]);
+            double ms = 0.0;
+            rc = edpp_batch_run(b, &ms);  // warm-up
+            for (int r = 0; r < reps && !rc; ++r) rc = edpp_batch_run(b, &ts[(size_t)r]);
+            if (rc) return rc;
+            std::sort(ts.begin(), ts.end());
+            const double med = ts[(size_t)reps / 2];
+            if (times) times[s * kNumOcc + o] = med;
+            if (best < 0 || med < best) {
+                best = med;
+                best_shape = s;
+                best_occ = kOccupancies[o];
+            }
+        }
+    }
+    int rc = edpp_batch_set_shape(b, best_shape);
+    if (!rc) rc = edpp_batch_set_occupancy(b, best_occ);
+    return rc;
+}
+
 int edpp_batch_destroy(edpp_batch* b) {
     if (!b) return EDPP_OK;
     if (b->ctx && b->ctx->stream) hipStreamSynchronize(b->ctx->stream);
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
+    drop_plan(b);
     if (b->cap_stream) hipStreamDestroy(b->cap_stream);
-    hipFree(b->state);
-    hipFree(b->rest);
-    hipFree(b->d_jobs);
-    hipFree(b->d_segs);
     delete b;
     return EDPP_OK;
 }

@@ -91,6 +91,8 @@ int launch_sort_u32(edpp_ctx* c, const uint32_t* kin, uint32_t* kout, const int3
 int launch_access_scale(edpp_ctx* c, int pattern, const int32_t* in, int32_t* out, int64_t n, int repeat);
 int launch_access_chunks(edpp_ctx* c, const int32_t* in, int64_t n, int64_t chunk, const int32_t* order,
                          unsigned long long* d_count);
+int launch_access_random(edpp_ctx* c, int32_t* table, int64_t table_n, int64_t accesses, uint32_t seed, bool write,
+                         unsigned long long* d_sum);
 int launch_join_count(edpp_ctx* c, const uint32_t* rk, int64_t rn, const uint32_t* sk, int64_t sn,
                       unsigned long long* d_count);
 int launch_gen_u32(edpp_ctx* c, uint32_t* out, int64_t n, uint64_t seed, uint32_t mask);

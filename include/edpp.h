@@ -218,6 +218,19 @@ int edpp_access_scale_i32(edpp_ctx* ctx, int pattern, const int32_t* in, int32_t
                           int repeat, double* ms);
 int edpp_access_chunks_i32(edpp_ctx* ctx, const int32_t* in, int64_t n, int64_t chunk,
                            const int32_t* order, int64_t* d_count, double* ms);
+/* edpp_access_random_i32: the random-access ceiling that gather and scatter
+ * are graded against.  `accesses` 4-byte accesses to table positions
+ * p_i = fmix32(i + seed) & (table_n - 1), i < accesses (fmix32: the MurmurHash3
+ * finaliser, a bijection of 32-bit words), computed in registers, so no index
+ * stream is read.  write = 0: *d_sum (device, int64) = sum over i of
+ * (uint32)table[p_i], modulo 2^64.  write = 1: table[p_i] = p_i for every i
+ * (d_sum may be NULL).  table_n is a power of two in [1, 2^31], accesses in
+ * [0, 2^32).  The probe is the sequential-index counterpart of the
+ * reference's cold/warm cache probe (opencl/test/test_access_wg.cpp:36-46):
+ * a table of 1 GiB prices random reads from HBM, 128 MiB from the Infinity
+ * Cache, 2 MiB from each XCD's L2. */
+int edpp_access_random_i32(edpp_ctx* ctx, int32_t* table, int64_t table_n, int64_t accesses,
+                           uint64_t seed, int write, int64_t* d_sum, double* ms);
 
 /* ---- partitioned hash join ----------------------------------------------
  * Replaces the reference's hashjoin (obsolete/OpenCL/phjImpl.cpp:14-66 with

@@ -54,6 +54,24 @@ int edpp_batch_run_graph(edpp_batch* b, double* ms);
 int edpp_batch_destroy(edpp_batch* b);
 /* Workgroups of the persistent launch per compute unit (tuning; default 4). */
 int edpp_batch_set_occupancy(edpp_batch* b, int32_t wgs_per_cu);
+/* Tile shapes of the persistent launch.  Shape k fixes the lanes (waves) per
+ * workgroup and the elements per tile of each kind: 32- and 64-bit scan
+ * tiles, split tiles (lanes x keys per lane) and gather tiles.  Shape 0 is
+ * the default.  Setting a shape re-plans the batch's work queue; results do
+ * not depend on the shape or the occupancy. */
+int edpp_batch_num_shapes(void);
+int edpp_batch_shape_info(int32_t shape, int32_t* block, int32_t* scan_tile32, int32_t* scan_tile64,
+                          int32_t* split_tile, int32_t* gather_tile);
+int edpp_batch_set_shape(edpp_batch* b, int32_t shape);
+int edpp_batch_get_shape(edpp_batch* b, int32_t* shape, int32_t* wgs_per_cu);
+/* Exhaustive autotune of one batch, as the reference's launch-parameter
+ * search tries every local and grid size of a split and keeps the fastest
+ * (opencl/test/test_split.cpp:273-358): every shape at 2, 4 and 8 workgroups
+ * per compute unit, one warm-up run and `reps` timed runs each (median); the
+ * fastest pair stays set.  times (may be NULL) receives
+ * edpp_batch_num_shapes() * 3 medians in ms, shape-major.  Runs the batch
+ * repeatedly, so jobs writing their own input are rejected (EDPP_EINVAL). */
+int edpp_batch_autotune(edpp_batch* b, int32_t reps, double* times);
 /* Queue statistics: total tiles, split histogram tiles, histogram-scan tiles. */
 int edpp_batch_info(edpp_batch* b, int64_t* tiles, int64_t* hist_tiles, int64_t* hscan_tiles);
 

@@ -72,3 +72,58 @@ def test_batch_occupancy_and_rejects(torch_gpu, ctx, edpp):
         edpp.Batch(ctx, [edpp.make_job(edpp.JOB_SPLIT, n, x, y, 4, buckets=512)])
     with pytest.raises(edpp.EdppError):
         edpp.Batch(ctx, [edpp.make_job(edpp.JOB_SCAN, n - 1, x[1:], y[1:], 4)])  # not 16-byte aligned
+
+
+def test_batch_every_shape_matches_oracle(torch_gpu, ctx, orc, edpp):
+    # every tile shape of the autotuner, at two occupancies, on one mixed batch
+    torch = torch_gpu
+    jm = importlib.import_module("efficient-dpp_amd.jobmix")
+    shapes = edpp.batch_shapes()
+    assert len(shapes) >= 4 and shapes[0]["split_tile"] == shapes[0]["block"] * 8
+    assert len({s["block"] for s in shapes}) >= 2  # waves per workgroup vary
+    plan, _ = jm.mix_plan(120, 1 << 21, lo=1, hi=1 << 17, seed=9)
+    plan[3]["n"] = 0
+    jobsets, bufs, arena = jm.build_mix(edpp, ctx, torch, plan, outputs=1)
+    batch = edpp.Batch(ctx, jobsets[0])
+    try:
+        for k in range(len(shapes)):
+            batch.set_shape(k)
+            for occ in (2, 8):
+                batch.set_occupancy(occ)
+                assert batch.get_shape() == (k, occ)
+                for bb in bufs:
+                    bb["out0"].fill_(-1)
+                batch.run(graph=(occ == 8))
+                torch.cuda.synchronize()
+                for bb in bufs:
+                    check_job(orc, bb, 0)
+    finally:
+        batch.close()
+
+
+def test_batch_autotune_picks_a_timed_shape(torch_gpu, ctx, orc, edpp):
+    torch = torch_gpu
+    jm = importlib.import_module("efficient-dpp_amd.jobmix")
+    plan, _ = jm.mix_plan(200, 1 << 22, lo=256, hi=1 << 18, seed=10)
+    jobsets, bufs, arena = jm.build_mix(edpp, ctx, torch, plan, outputs=1)
+    batch = edpp.Batch(ctx, jobsets[0])
+    try:
+        times = batch.autotune(reps=2)
+        assert len(times) == len(edpp.batch_shapes()) * len(edpp.BATCH_OCCUPANCIES)
+        assert all(t > 0 for t in times.values())
+        chosen = batch.get_shape()
+        assert times[chosen] == min(times.values())
+        for bb in bufs:
+            bb["out0"].fill_(-1)
+        batch.run(graph=True)
+        torch.cuda.synchronize()
+        for bb in bufs:
+            check_job(orc, bb, 0)
+        # a job writing its own input cannot be autotuned
+        x = torch.ones(1000, dtype=torch.int32, device="cuda")
+        inplace = edpp.Batch(ctx, [edpp.make_job(edpp.JOB_SCAN, 1000, x, x, 4)])
+        with pytest.raises(edpp.EdppError):
+            inplace.autotune(1)
+        inplace.close()
+    finally:
+        batch.close()

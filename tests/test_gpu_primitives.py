@@ -551,3 +551,46 @@ def test_access_chunks(torch_gpu, ctx, orc, n, chunk):
                   np.random.default_rng(chunks).permutation(chunks)):
         ctx.access_chunks(dev(torch, a), chunk, dev(torch, np.ascontiguousarray(order, np.int32)), cnt)
         assert int(cnt.item()) == int((a & 1).sum()), (n, chunk)
+
+
+def _fmix32(x):
+    # MurmurHash3's finaliser on uint32 numpy arrays (the probe's position hash)
+    x = x.astype(np.uint32)
+    x ^= x >> np.uint32(16)
+    x *= np.uint32(0x85EBCA6B)
+    x ^= x >> np.uint32(13)
+    x *= np.uint32(0xC2B2AE35)
+    x ^= x >> np.uint32(16)
+    return x
+
+
+@pytest.mark.parametrize("table_n,accesses", [(1, 5), (64, 1000), (1 << 12, 1 << 16), (1 << 20, 3_000_001),
+                                              (1 << 16, 0)])
+def test_access_random_probe(torch_gpu, ctx, orc, table_n, accesses):
+    torch = torch_gpu
+    seed = 0x9E37 + table_n
+    tab = orc.gen_i32(table_n, 17 + table_n)
+    with np.errstate(over="ignore"):
+        pos = _fmix32(np.arange(accesses, dtype=np.uint64).astype(np.uint32) + np.uint32(seed & 0xFFFFFFFF))
+    pos &= np.uint32(table_n - 1)
+    want = int(tab.view(np.uint32)[pos].astype(np.uint64).sum()) & (2**64 - 1)
+    dt = dev(torch, tab)
+    tot = torch.full((1,), -1, dtype=torch.int64, device="cuda")
+    ctx.access_random(dt, accesses, seed, False, tot)
+    assert int(tot.item()) & (2**64 - 1) == want
+    # writes: every touched position holds its own index, the others keep -1
+    wt = torch.full((table_n,), -1, dtype=torch.int32, device="cuda")
+    ctx.access_random(wt, accesses, seed, True)
+    exp = np.full(table_n, -1, np.int32)
+    exp[pos] = pos.astype(np.int32)
+    assert np.array_equal(host(wt), exp)
+
+
+def test_access_random_rejects(torch_gpu, ctx, edpp):
+    torch = torch_gpu
+    t = torch.zeros(100, dtype=torch.int32, device="cuda")
+    tot = torch.zeros(1, dtype=torch.int64, device="cuda")
+    with pytest.raises(edpp.EdppError):
+        ctx.access_random(t, 10, 1, False, tot)  # 100 is not a power of two
+    with pytest.raises(edpp.EdppError):
+        ctx.access_random(t[:64], 1 << 32, 1, False, tot, table_n=64)
