@@ -55,6 +55,7 @@ _SIGS = {
     "edpp_sort_i32": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_dp],
     "edpp_scan_exclusive_i32": [c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_dp],
     "edpp_scan_exclusive_i64": [c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_dp],
+    "edpp_scan_exclusive_i32_noalloc": [c_vp, c_vp, c_vp, c_i64, c_vp, c_dp],
     "edpp_scan_local_i32": [c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_dp],
     "edpp_split_i32": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_dp],
     "edpp_split_i64": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_dp],
@@ -380,6 +381,14 @@ class Context:
         w = 8 if wide else 4
         return self._call(name, _buf(src, n, w, "scan in"), _buf(dst, n, w, "scan out"), n, int(scheme),
                           _buf(total, 1, w, "scan total"), timed=timed)
+
+    def scan_noalloc(self, src, dst, n=None, total=None, timed=False):
+        """Exclusive int32 scan that allocates nothing: its look-back state is
+        the library's fixed per-GPU scratch array (calls on one GPU must not
+        overlap)."""
+        n = _count(_numel(src) if n is None else n, "scan_noalloc")
+        return self._call("edpp_scan_exclusive_i32_noalloc", _buf(src, n, 4, "scan in"), _buf(dst, n, 4, "scan out"),
+                          n, _buf(total, 1, 4, "scan total"), timed=timed)
 
     def scan_local(self, src, dst, tile_size, local_size=512, matrix=MATRIX_LM_REG, network=NET_WAVE, timed=False):
         n = max(int(tile_size), 0) * max(int(local_size), 0)

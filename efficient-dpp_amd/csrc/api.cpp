@@ -234,6 +234,8 @@ int edpp_ctx_create(int device, void* stream, edpp_ctx** out) {
             rc = EDPP_EHIP;
             break;
         }
+        rc = noalloc_setup(c);
+        if (rc) break;
     } while (0);
     if (rc) {
         edpp_ctx_destroy(c);

@@ -33,6 +33,9 @@ struct edpp_ctx {
     uint32_t epoch = 1;                 // 30-bit, 0 never used
     uint32_t spin_limit = 1u << 22;     // look-back watchdog (polls before giving up, seconds)
 
+    // this device's copy of the no-allocation scan's fixed scratch array
+    uint64_t* noalloc_scratch = nullptr;
+
     edpp_tuning tune = default_tuning();
     edpp_mlp* model = nullptr;          // when set, AUTO calls ask the cost model
     std::string last_error;
@@ -81,6 +84,9 @@ template <class T>
 int launch_scan(edpp_ctx* c, const T* in, T* out, int64_t n, int scheme, T* d_total);
 template <class T>
 int launch_gather(edpp_ctx* c, const T* in, T* out, const int32_t* idx, int64_t n, int passes, bool scatter);
+int noalloc_setup(edpp_ctx* c);
+int launch_scan_lookback_u32_on(edpp_ctx* c, const uint32_t* in, uint32_t* out, int64_t n, uint32_t* d_total,
+                                uint64_t* status, int64_t status_cap, uint32_t* tickets);
 int launch_scan_local(edpp_ctx* c, const int32_t* in, int32_t* out, int tile, int local_size,
                       int matrix_type, int network);
 template <class K>

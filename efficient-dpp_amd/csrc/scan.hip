@@ -322,6 +322,41 @@ int launch_scan(edpp_ctx* c, const T* in, T* out, int64_t n, int scheme, T* d_to
     }
 }
 
+// Look-back scan of 32-bit values (the default tile shape, or the scalar
+// chunks for unaligned buffers) on state the caller provides: `status` holds
+// status_cap words and `tickets` two counters.  Both are cleared here by a
+// fill kernel and the epoch is fixed, so nothing is allocated and nothing is
+// kept from one call to the next.
+int launch_scan_lookback_u32_on(edpp_ctx* c, const uint32_t* in, uint32_t* out, int64_t n, uint32_t* d_total,
+                                uint64_t* status, int64_t status_cap, uint32_t* tickets) {
+    if (n == 0) return d_total ? zero_fill(c, d_total, sizeof(uint32_t)) : EDPP_OK;
+    const bool vec = aligned16(in) && aligned16(out);
+    int block = 1024, vpt = 8;
+    default_scan_shape(c, n, &block, &vpt);
+    const int64_t tile = vec ? (int64_t)block * 4 * vpt : 64 * 16;
+    const int64_t tiles = ceil_div(n, tile);
+    if (tiles > status_cap)
+        return fail(c, EDPP_EINVAL, "n=%lld needs %lld look-back words, more than the %lld available", (long long)n,
+                    (long long)tiles, (long long)status_cap);
+    int rc = zero_fill(c, status, (size_t)tiles * sizeof(uint64_t));
+    if (!rc) rc = zero_fill(c, tickets, 2 * sizeof(uint32_t));
+    if (rc) return rc;
+    LookbackState32 st;
+    st.status = status;
+    st.epoch = 1;
+    if (!vec)
+        hipLaunchKernelGGL((k_scan_unaligned<uint32_t>), dim3((unsigned)tiles), dim3(64), 0, c->stream, in, out, n, st,
+                           tickets, d_total, c->spin_limit, c->d_err, tiles);
+    else if (vpt <= 4)
+        hipLaunchKernelGGL((k_scan_lookback<uint32_t, 1024, 4>), dim3((unsigned)tiles), dim3(1024), 0, c->stream, in,
+                           out, n, st, tickets, d_total, c->spin_limit, c->d_err, tiles);
+    else
+        hipLaunchKernelGGL((k_scan_lookback<uint32_t, 1024, 8>), dim3((unsigned)tiles), dim3(1024), 0, c->stream, in,
+                           out, n, st, tickets, d_total, c->spin_limit, c->d_err, tiles);
+    EDPP_HIP(c, hipGetLastError());
+    return EDPP_OK;
+}
+
 template int launch_scan<uint32_t>(edpp_ctx*, const uint32_t*, uint32_t*, int64_t, int, uint32_t*);
 template int launch_scan<uint64_t>(edpp_ctx*, const uint64_t*, uint64_t*, int64_t, int, uint64_t*);
 

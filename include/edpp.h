@@ -159,6 +159,18 @@ int edpp_scan_exclusive_i32(edpp_ctx* ctx, const int32_t* in, int32_t* out, int6
                             int scheme, int32_t* d_total, double* ms);
 int edpp_scan_exclusive_i64(edpp_ctx* ctx, const int64_t* in, int64_t* out, int64_t n,
                             int scheme, int64_t* d_total, double* ms);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the look-back scan of edpp_scan_exclusive_i32 (the same
+ * output as every scheme of it) whose look-back state lives in a fixed device
+ * array of 512 KiB + 256 B per GPU, held until the process ends, instead of
+ * the context's workspace.  The call allocates and frees nothing.  Limits:
+ * n < 2^31 when in and out are 16-byte aligned, n <= 2^26 otherwise
+ * (EDPP_EINVAL beyond).  Every call on one GPU shares that array and clears
+ * the part it uses first: calls that overlap on one GPU (other streams,
+ * contexts or threads) race and must not be issued.                        */
+int edpp_scan_exclusive_i32_noalloc(edpp_ctx* ctx, const int32_t* in, int32_t* out, int64_t n,
+                                    int32_t* d_total, double* ms);
 
 /* ---- single-workgroup tile scans ----------------------------------------
  * opencl/kernels/scan_local_kernel.cl:478-695, driver
