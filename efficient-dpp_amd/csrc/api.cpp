@@ -280,7 +280,8 @@ int edpp_set_tuning(edpp_ctx* c, const edpp_tuning* t) {
         (v.scan_vpt != 2 && v.scan_vpt != 4 && v.scan_vpt != 8 && v.scan_vpt != 16) || (v.split_ipt != 8 && v.split_ipt != 16) ||
         v.split_tile_mult < 16 || v.split_tile_mult > 1024 ||
         (v.scan_block != 256 && v.scan_block != 512 && v.scan_block != 1024) ||
-        (v.split_block != 0 && v.split_block != 256 && v.split_block != 512))
+        (v.split_block != 0 && v.split_block != 256 && v.split_block != 512) ||
+        v.split_reorder < EDPP_REORDER_DEFAULT || v.split_reorder > EDPP_REORDER_VARIED)
         return fail(c, EDPP_EINVAL, "tuning value out of range");
     c->tune = v;
     return EDPP_OK;

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <mutex>
 #include "edpp_primitives.hpp"
+#include "edpp_tune.h"
 
 namespace edpp {
 
@@ -107,10 +108,21 @@ double WG_split(const int* d_in, int* d_out, int* d_start, int length, int bucke
                 DataStruc structure, const int* d_in_values, int* d_out_values, int local_size, int grid_size) {
     (void)local_size;
     (void)grid_size;
-    // every reorder variant is an alias of the stable WG split with its LDS
-    // reorder buffer (see the header): same output, one schedule
-    (void)reorder_type;
-    return split_common(d_in, d_out, d_start, length, buckets, structure, d_in_values, d_out_values, EDPP_SPLIT_WG);
+    // the reorder type picks how the WG split writes its tiles; every variant
+    // gives the same stable output
+    edpp_ctx* c = context();
+    if (!c) return -1.0;
+    if (reorder_type != NO_REORDER && reorder_type != FIXED_REORDER && reorder_type != VARIED_REORDER) return -1;
+    edpp_tuning saved, t;
+    edpp_get_tuning(c, &saved);
+    t = saved;
+    t.split_reorder = reorder_type == NO_REORDER ? EDPP_REORDER_NONE
+                      : reorder_type == FIXED_REORDER ? EDPP_REORDER_FIXED : EDPP_REORDER_VARIED;
+    if (edpp_set_tuning(c, &t) != EDPP_OK) return -1.0;
+    const double ms =
+        split_common(d_in, d_out, d_start, length, buckets, structure, d_in_values, d_out_values, EDPP_SPLIT_WG);
+    edpp_set_tuning(c, &saved);
+    return ms;
 }
 
 double single_split(const int* d_in, int* d_out, int length, int buckets, bool reorder, DataStruc structure) {

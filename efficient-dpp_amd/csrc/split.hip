@@ -61,17 +61,40 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_hist(const K* __restrict_
     }
 }
 
+// Reorder variants of the scatter (the reference's ReorderType,
+// opencl/types.h:43-45): each bucket's run of a sub-tile staged whole in LDS
+// (varied, the default), elements stored straight from registers (none), or
+// fixed-length per-bucket staging buffers flushed when they fill (fixed).
+enum : int { kReorderVaried = 0, kReorderNone = 1, kReorderFixed = 2 };
+
+// LDS words in front of the fixed variant's staging buffers: the varied
+// layout of kbuf_word_offset plus a sub-tile of keys and values
+template <class K>
+__host__ __device__ constexpr int fixed_ccnt_word_offset(int B, int S) {
+    return kbuf_word_offset(B) + (int)((S * (sizeof(K) + 4) + 3) / 4);
+}
+template <class K>
+__host__ __device__ constexpr int fixed_ck_word_offset(int B, int S) {
+    return (fixed_ccnt_word_offset<K>(B, S) + B + 3) & ~3;  // 16-byte aligned
+}
+template <class K>
+__host__ __device__ constexpr size_t fixed_lds_bytes(int B, int S) {
+    return (size_t)fixed_ck_word_offset<K>(B, S) * 4 + (size_t)B * kFixedCap * (sizeof(K) + 4);
+}
+
 // BLOCK threads; sub-tile S = BLOCK*IPT.  Above 256 threads the values are
 // staged after the keys through one buffer (place_and_store's SEP), and the
 // register budget is held to two workgroups per CU (4 waves per SIMD), which
-// their LDS allows.
-template <class K, int IPT, bool KV, bool BALLOT, int BLOCK>
+// their LDS allows.  RO: kReorder*; the none and fixed variants run with
+// 256 lanes.
+template <class K, int IPT, bool KV, bool BALLOT, int BLOCK, int RO = kReorderVaried>
 __global__ __launch_bounds__(BLOCK, (BLOCK > kSplitBlock ? 2 * BLOCK / (4 * kWave) : 1)) void k_split_scatter(const K* __restrict__ kin, K* __restrict__ kout,
                                                          const int32_t* __restrict__ vin, int32_t* __restrict__ vout,
                                                          SplitGeom g, const int32_t* __restrict__ scanned) {
     extern __shared__ uint32_t lds[];
     constexpr int W = BLOCK / kWave;
     constexpr bool SEP = KV && BLOCK > kSplitBlock;
+    static_assert(RO == kReorderVaried || BLOCK == kSplitBlock, "the none and fixed variants use 256 lanes");
     const int B = 1 << g.lg;
     constexpr int S = BLOCK * IPT;
     uint64_t* wmask = reinterpret_cast<uint64_t*>(lds);  // [waves][B], zero between items
@@ -86,7 +109,13 @@ __global__ __launch_bounds__(BLOCK, (BLOCK > kSplitBlock ? 2 * BLOCK / (4 * kWav
     const int64_t tbase = tile * g.tile;
     const int64_t tend = std::min<int64_t>(tbase + g.tile, g.n);
     const int w = threadIdx.x / kWave;
-    for (int b = threadIdx.x; b < B; b += BLOCK) run[b] = (uint32_t)scanned[(int64_t)b * g.tiles + tile];
+    uint32_t* ccnt = lds + fixed_ccnt_word_offset<K>(B, S);             // fixed: [B] held elements
+    K* ck = reinterpret_cast<K*>(lds + fixed_ck_word_offset<K>(B, S));  // fixed: [B][kFixedCap] held keys
+    int32_t* cv = reinterpret_cast<int32_t*>(ck + B * kFixedCap);       // fixed: held values
+    for (int b = threadIdx.x; b < B; b += BLOCK) {
+        run[b] = (uint32_t)scanned[(int64_t)b * g.tiles + tile];
+        if (RO == kReorderFixed) ccnt[b] = 0;
+    }
     for (int b = threadIdx.x; b < W * B; b += BLOCK) wmask[b] = 0ull;
     for (int64_t s = tbase; s < tend; s += S) {
         for (int b = threadIdx.x; b < W * B; b += BLOCK) whist[b] = 0;
@@ -97,10 +126,21 @@ __global__ __launch_bounds__(BLOCK, (BLOCK > kSplitBlock ? 2 * BLOCK / (4 * kWav
         rank_subtile<K, IPT, BALLOT>(kin, KV ? vin : nullptr, s, tend, g.lg, g.shift, whist + w * B, wmask + w * B,
                                      bk, rk, key, val, g.flip);
         __syncthreads();
-        place_and_store<K, IPT, KV, BLOCK, SEP>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk, key, val, B, g.shift,
-                                                (int)std::min<int64_t>((int64_t)S, tend - s), kout, vout, g.n, g.flip);
+        const int cnt = (int)std::min<int64_t>((int64_t)S, tend - s);
+        if (RO == kReorderNone)
+            store_direct<K, IPT, KV, BLOCK>(whist, run, sub, bk, rk, key, val, B, kout, vout, g.n);
+        else if (RO == kReorderFixed)
+            place_fixed<K, IPT, KV, BLOCK>(whist, run, sub, off, wtmp, kbuf, vbuf, ck, cv, ccnt, bk, rk, key, val, B,
+                                           g.shift, cnt, kout, vout, g.n, g.flip);
+        else
+            place_and_store<K, IPT, KV, BLOCK, SEP>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk, key, val, B,
+                                                    g.shift, cnt, kout, vout, g.n, g.flip);
         // the next iteration's first barrier ends every thread's stores
         // before kbuf, sub and run are used again
+    }
+    if (RO == kReorderFixed) {
+        __syncthreads();
+        flush_fixed<K, KV, BLOCK>(run, ck, cv, ccnt, B, kout, vout, g.n);
     }
 }
 
@@ -264,9 +304,16 @@ int64_t single_lanes(edpp_ctx* c, int64_t n, int lg) {
     return lanes;
 }
 
+bool reorder_256(edpp_ctx* c) {
+    return c->tune.split_reorder == EDPP_REORDER_NONE || c->tune.split_reorder == EDPP_REORDER_FIXED;
+}
+
 template <class K>
 int wg_ipt(edpp_ctx* c, int lg) {
-    return (c->tune.split_ipt >= 16 && wg_lds_bytes<16, K>(1 << lg) <= 160 * 1024) ? 16 : 8;
+    if (c->tune.split_ipt < 16 || wg_lds_bytes<16, K>(1 << lg) > 160 * 1024) return 8;
+    if (c->tune.split_reorder == EDPP_REORDER_FIXED && fixed_lds_bytes<K>(1 << lg, kSplitBlock * 16) > 160 * 1024)
+        return 8;
+    return 16;
 }
 
 // Lanes per scatter workgroup.  512 (sub-tiles of 8192 keys, values staged
@@ -276,6 +323,7 @@ int wg_ipt(edpp_ctx* c, int lg) {
 // 1.22 ms uniform, 1.16 against 1.26 ms Zipf, 2^28 pairs).
 template <class K>
 int wg_block(edpp_ctx* c, int lg) {
+    if (reorder_256(c)) return kSplitBlock;
     if (sizeof(K) != 4 || wg_ipt<K>(c, lg) != 16 || wg_lds_bytes<16, K, 512>(1 << lg) > 160 * 1024) return kSplitBlock;
     if (c->tune.split_block != 0) return c->tune.split_block;
     return lg >= 8 ? 512 : kSplitBlock;
@@ -290,7 +338,7 @@ int64_t his_len(edpp_ctx* c, int strategy, int64_t n, int lg) {
     return B * single_lanes(c, n, lg);
 }
 
-template <class K, int IPT, bool BALLOT, int BLOCK>
+template <class K, int IPT, bool BALLOT, int BLOCK, int RO = kReorderVaried>
 int run_wg(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
     const int B = 1 << a.lg;
     constexpr int S = BLOCK * IPT;         // scatter sub-tile
@@ -311,10 +359,12 @@ int run_wg(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
     const bool multi = regw == 0 && spt <= kHistTilesPerWG && (spt & (spt - 1)) == 0;
     const int tpw = multi ? kHistTilesPerWG / (int)spt : 1;
     const size_t lh = (size_t)tpw * kSplitWaves * B * 4;
-    const size_t ls = wg_lds_bytes<IPT, K, BLOCK>(B);
-    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_scatter<K, IPT, true, BALLOT, BLOCK>,
+    const size_t ls = RO == kReorderFixed ? fixed_lds_bytes<K>(B, S)
+                      : RO == kReorderNone ? (size_t)kbuf_word_offset(B) * 4
+                                           : wg_lds_bytes<IPT, K, BLOCK>(B);
+    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_scatter<K, IPT, true, BALLOT, BLOCK, RO>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls));
-    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_scatter<K, IPT, false, BALLOT, BLOCK>,
+    EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_scatter<K, IPT, false, BALLOT, BLOCK, RO>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls));
     const unsigned hgrid = (unsigned)ceil_div(g.tiles, (int64_t)tpw);
     const int ispt = (int)spt;
@@ -337,11 +387,11 @@ int run_wg(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
         hipLaunchKernelGGL(k_split_start, dim3((unsigned)ceil_div(B, 256)), dim3(256), 0, c->stream,
                            (const int32_t*)his, g.tiles, B, a.d_start);
     if (a.vin)
-        hipLaunchKernelGGL((k_split_scatter<K, IPT, true, BALLOT, BLOCK>), dim3((unsigned)g.tiles), dim3(BLOCK), ls,
-                           c->stream, a.kin, a.kout, a.vin, a.vout, g, (const int32_t*)his);
+        hipLaunchKernelGGL((k_split_scatter<K, IPT, true, BALLOT, BLOCK, RO>), dim3((unsigned)g.tiles), dim3(BLOCK),
+                           ls, c->stream, a.kin, a.kout, a.vin, a.vout, g, (const int32_t*)his);
     else
-        hipLaunchKernelGGL((k_split_scatter<K, IPT, false, BALLOT, BLOCK>), dim3((unsigned)g.tiles), dim3(BLOCK), ls,
-                           c->stream, a.kin, a.kout, a.vin, a.vout, g, (const int32_t*)his);
+        hipLaunchKernelGGL((k_split_scatter<K, IPT, false, BALLOT, BLOCK, RO>), dim3((unsigned)g.tiles), dim3(BLOCK),
+                           ls, c->stream, a.kin, a.kout, a.vin, a.vout, g, (const int32_t*)his);
     EDPP_HIP(c, hipGetLastError());
     return EDPP_OK;
 }
@@ -409,6 +459,23 @@ int run_pass(edpp_ctx* c, int strategy, const SplitCall<K>& a, int32_t* his) {
     switch (strategy) {
         case EDPP_SPLIT_WG: {
             const bool bal = a.lg <= kBallotMatchBits;
+            const int ro = c->tune.split_reorder;
+            if (ro == EDPP_REORDER_NONE || ro == EDPP_REORDER_FIXED) {
+                // 256 lanes (wg_block); wg_ipt also checks that the fixed
+                // variant's staging fits beside 16 keys per lane
+                const bool fx = ro == EDPP_REORDER_FIXED;
+                const bool i16 = wg_ipt<K>(c, a.lg) == 16;
+                if (fx) {
+                    if (i16) return bal ? run_wg<K, 16, true, kSplitBlock, kReorderFixed>(c, a, his)
+                                        : run_wg<K, 16, false, kSplitBlock, kReorderFixed>(c, a, his);
+                    return bal ? run_wg<K, 8, true, kSplitBlock, kReorderFixed>(c, a, his)
+                               : run_wg<K, 8, false, kSplitBlock, kReorderFixed>(c, a, his);
+                }
+                if (i16) return bal ? run_wg<K, 16, true, kSplitBlock, kReorderNone>(c, a, his)
+                                    : run_wg<K, 16, false, kSplitBlock, kReorderNone>(c, a, his);
+                return bal ? run_wg<K, 8, true, kSplitBlock, kReorderNone>(c, a, his)
+                           : run_wg<K, 8, false, kSplitBlock, kReorderNone>(c, a, his);
+            }
             if (wg_ipt<K>(c, a.lg) == 16) {
                 if constexpr (sizeof(K) == 4) {
                     if (wg_block<K>(c, a.lg) == 512)

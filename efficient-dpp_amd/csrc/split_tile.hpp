@@ -460,4 +460,161 @@ __device__ __forceinline__ void place_and_store(uint32_t* whist, uint32_t* run, 
     }
 }
 
+// ------------------------------------------------------------- reorder variants
+// The reference's WG split offers three ways to write a tile's elements
+// (opencl/types.h:43-45, opencl/kernels/split_kernel.cl:455-665): straight
+// to their output positions (NO_REORDER), through per-bucket staging buffers
+// of a fixed length flushed whenever they fill (FIXED_REORDER), or through a
+// buffer holding each bucket's whole run of the tile (VARIED_REORDER, which
+// place_and_store above implements).  Both functions below give the same
+// stable output.
+
+// NO_REORDER: each element goes from registers to run[b] + (elements of b in
+// earlier waves of the sub-tile) + rk.  whist holds the per-wave counts on
+// entry; run[b] advances by the sub-tile's count of b.  Ends without a
+// barrier after the run update (the caller's next barrier orders it).
+template <class K, int IPT, bool KV, int BLOCK = kSplitBlock>
+__device__ __forceinline__ void store_direct(uint32_t* whist, uint32_t* run, uint32_t* sub,
+                                             const uint32_t (&bk)[IPT], const uint32_t (&rk)[IPT],
+                                             const K (&key)[IPT], const int32_t (&val)[IPT], int B,
+                                             K* __restrict__ kout, int32_t* __restrict__ vout, int64_t n) {
+    constexpr int W = BLOCK / kWave;
+    const int w = threadIdx.x / kWave;
+    for (int b = threadIdx.x; b < B; b += BLOCK) {
+        uint32_t x[W];
+        #pragma unroll
+        for (int q = 0; q < W; ++q) x[q] = whist[q * B + b];
+        uint32_t a = 0;
+        #pragma unroll
+        for (int q = 0; q < W; ++q) {
+            whist[q * B + b] = a;
+            a += x[q];
+        }
+        sub[b] = a;
+    }
+    __syncthreads();
+    #pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        if (bk[j] != 0xFFFFFFFFu) {
+            const uint32_t p = run[bk[j]] + whist[w * B + bk[j]] + rk[j];
+            if ((int64_t)p < n) {  // always true for a consistent histogram scan
+                kout[p] = key[j];
+                if (KV) vout[p] = val[j];
+            }
+        }
+    }
+    __syncthreads();  // every read of run[] before it advances
+    for (int b = threadIdx.x; b < B; b += BLOCK) run[b] += sub[b];
+}
+
+// FIXED_REORDER: per-bucket staging buffers of kFixedCap elements in LDS
+// (ck/cv, [B][kFixedCap], ccnt[b] elements held).  The sub-tile is first
+// reordered by bucket in kbuf/vbuf as in place_and_store; then, for every
+// bucket, the held elements followed by the sub-tile's run form one run of
+// which the largest multiple of kFixedCap is written to run[b] onwards and
+// the rest is held for the next sub-tile.  flush_fixed writes what is held
+// at the end of the tile.  run[b] is the output position of the first held
+// element.  Ends with the per-bucket updates after a barrier (the caller's
+// next barrier orders them).
+constexpr int kFixedCap = 16;
+
+template <class K, int IPT, bool KV, int BLOCK = kSplitBlock>
+__device__ __forceinline__ void place_fixed(uint32_t* whist, uint32_t* run, uint32_t* sub, uint32_t* off,
+                                            uint32_t* wtmp, K* kbuf, int32_t* vbuf, K* ck, int32_t* cv,
+                                            uint32_t* ccnt, const uint32_t (&bk)[IPT], const uint32_t (&rk)[IPT],
+                                            const K (&key)[IPT], const int32_t (&val)[IPT], int B, int shift,
+                                            int cnt, K* __restrict__ kout, int32_t* __restrict__ vout, int64_t n,
+                                            uint32_t flip = 0) {
+    constexpr int W = BLOCK / kWave;
+    constexpr int S = BLOCK * IPT;
+    const int w = threadIdx.x / kWave;
+    const uint32_t mask = (uint32_t)B - 1u;
+    for (int b = threadIdx.x; b < B; b += BLOCK) {
+        uint32_t x[W];
+        #pragma unroll
+        for (int q = 0; q < W; ++q) x[q] = whist[q * B + b];
+        uint32_t a = 0;
+        #pragma unroll
+        for (int q = 0; q < W; ++q) {
+            whist[q * B + b] = a;
+            a += x[q];
+        }
+        sub[b] = a;
+    }
+    __syncthreads();
+    scan_buckets<BLOCK>(sub, off, B, wtmp);
+    __syncthreads();
+    #pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        if (bk[j] != 0xFFFFFFFFu) {
+            const uint32_t pos = off[bk[j]] + whist[w * B + bk[j]] + rk[j];
+            if (pos < (uint32_t)S) {
+                kbuf[pos] = key[j];
+                if (KV) vbuf[pos] = val[j];
+            }
+        }
+    }
+    __syncthreads();
+    // flush: combined index ci of a bucket's element (held ones first);
+    // fl = whole staging lengths of the combined run, written now
+    for (int i = threadIdx.x; i < cnt; i += BLOCK) {
+        const K k = kbuf[i];
+        const uint32_t b = bucket_of(k, shift, mask, flip);
+        const uint32_t ci = ccnt[b] + ((uint32_t)i - off[b]);
+        const uint32_t fl = (ccnt[b] + sub[b]) / kFixedCap * kFixedCap;
+        if (ci < fl) {
+            const uint32_t p = run[b] + ci;
+            if ((int64_t)p < n) {
+                kout[p] = k;
+                if (KV) vout[p] = vbuf[i];
+            }
+        }
+    }
+    for (int e = threadIdx.x; e < B * kFixedCap; e += BLOCK) {
+        const int b = e / kFixedCap, k = e % kFixedCap;
+        const uint32_t fl = (ccnt[b] + sub[b]) / kFixedCap * kFixedCap;
+        if ((uint32_t)k < ccnt[b] && (uint32_t)k < fl) {  // fl > 0 flushes every held element
+            const uint32_t p = run[b] + (uint32_t)k;
+            if ((int64_t)p < n) {
+                kout[p] = ck[e];
+                if (KV) vout[p] = cv[e];
+            }
+        }
+    }
+    __syncthreads();  // held elements read before the new ones take their slots
+    for (int i = threadIdx.x; i < cnt; i += BLOCK) {
+        const K k = kbuf[i];
+        const uint32_t b = bucket_of(k, shift, mask, flip);
+        const uint32_t ci = ccnt[b] + ((uint32_t)i - off[b]);
+        const uint32_t fl = (ccnt[b] + sub[b]) / kFixedCap * kFixedCap;
+        if (ci >= fl) {  // held: ci - fl < kFixedCap
+            ck[b * kFixedCap + (ci - fl)] = k;
+            if (KV) cv[b * kFixedCap + (ci - fl)] = vbuf[i];
+        }
+    }
+    __syncthreads();  // every read of ccnt, sub and run before they change
+    for (int b = threadIdx.x; b < B; b += BLOCK) {
+        const uint32_t fl = (ccnt[b] + sub[b]) / kFixedCap * kFixedCap;
+        ccnt[b] = ccnt[b] + sub[b] - fl;
+        run[b] += fl;
+    }
+}
+
+// FIXED_REORDER, end of a tile: every held element to run[b] onwards
+template <class K, bool KV, int BLOCK = kSplitBlock>
+__device__ __forceinline__ void flush_fixed(const uint32_t* run, const K* ck, const int32_t* cv,
+                                            const uint32_t* ccnt, int B, K* __restrict__ kout,
+                                            int32_t* __restrict__ vout, int64_t n) {
+    for (int e = threadIdx.x; e < B * kFixedCap; e += BLOCK) {
+        const int b = e / kFixedCap, k = e % kFixedCap;
+        if ((uint32_t)k < ccnt[b]) {
+            const uint32_t p = run[b] + (uint32_t)k;
+            if ((int64_t)p < n) {
+                kout[p] = ck[e];
+                if (KV) vout[p] = cv[e];
+            }
+        }
+    }
+}
+
 }  // namespace edpp

@@ -71,11 +71,16 @@ enum edpp_matrix_scan {          /* MatrixScanType, opencl/types.h:12-14 */
 };
 
 enum edpp_local_network {        /* intra-workgroup scan of the per-thread partials,
-                                    opencl/kernels/scan_local_kernel.cl:42-259 */
-    EDPP_NET_SERIAL   = 0,       /* lane 0 walks the partials (the reference's active choice, :504) */
-    EDPP_NET_BLELLOCH = 1,       /* work-efficient up/down sweep in LDS                           */
-    EDPP_NET_SKLANSKY = 2,       /* log-depth Sklansky in LDS                                      */
-    EDPP_NET_WAVE     = 3        /* 64-lane DPP wave scan + LDS across waves                       */
+                                    opencl/kernels/scan_local_kernel.cl:42-474 */
+    EDPP_NET_SERIAL          = 0, /* lane 0 walks the partials (the reference's active choice, :504)   */
+    EDPP_NET_BLELLOCH        = 1, /* work-efficient up/down sweep in LDS (:103-137)                      */
+    EDPP_NET_SKLANSKY        = 2, /* log-depth Sklansky in LDS (:181-211)                                */
+    EDPP_NET_WAVE            = 3, /* 64-lane DPP wave scan + LDS across waves (the warp scan, :214-259)  */
+    EDPP_NET_SERIAL_MT       = 4, /* serial runs on 128 lanes, lane 0 scans their sums (:57-100)         */
+    EDPP_NET_BLELLOCH_CF     = 5, /* Blelloch on a padded LDS copy, one pad word per 32 (:140-178)       */
+    EDPP_NET_GLOBAL_BLELLOCH = 6, /* Blelloch on a global-memory buffer, the output's head (:437-474)    */
+    EDPP_NET_GLOBAL_SKLANSKY = 7, /* Sklansky on a global-memory buffer (:403-435)                       */
+    EDPP_NET_GLOBAL_WAVE     = 8  /* wave scans, wave totals through global memory (:309-401)            */
 };
 
 enum edpp_split_strategy {       /* SPLIT_ALGO / ReorderType, opencl/types.h:32-42 */

@@ -58,13 +58,13 @@ double matrix_scan(const int* d_in, int* d_out, int tile_size, MatrixScanType ty
 // int pairs and the value arguments must be null.
 double WI_split(const int* d_in, int* d_out, int* d_start, int length, int buckets, DataStruc structure,
                 const int* d_in_values = 0, int* d_out_values = 0, int local_size = 256, int grid_size = 32768);
-// opencl/primitives.h:39-44.  reorder_type is accepted and ignored: NO_REORDER,
-// FIXED_REORDER and VARIED_REORDER all run the one stable WG split
-// (EDPP_SPLIT_WG), whose LDS reorder buffer holds one variable-length run per
-// bucket like VARIED_REORDER (opencl/kernels/split_kernel.cl:497-566).  The
-// reference's FIXED_REORDER (split_kernel.cl:574-665) stages each bucket in a
-// fixed-length host-memory buffer for CPUs and MICs; its output is the same
-// stable split, so the alias changes timing only, never results.
+// opencl/primitives.h:39-44.  reorder_type selects how the stable WG split
+// (EDPP_SPLIT_WG) writes a tile, as the reference's kernels do
+// (opencl/kernels/split_kernel.cl:455-665): NO_REORDER stores each element
+// straight to its output position, FIXED_REORDER stages each bucket in a
+// fixed-length LDS buffer written whenever it fills, VARIED_REORDER stages
+// each bucket's whole run of a sub-tile (the library's default).  All three
+// give the same stable output; any other value returns -1.
 double WG_split(const int* d_in, int* d_out, int* d_start, int length, int buckets, ReorderType reorder_type,
                 DataStruc structure, const int* d_in_values = 0, int* d_out_values = 0, int local_size = 256,
                 int grid_size = 32768);

@@ -27,8 +27,18 @@ typedef struct edpp_tuning {
     int32_t scan_block;       /* lanes per look-back workgroup: 256, 512, 1024; 0 = default              */
     int32_t split_block;      /* lanes per WG-split scatter workgroup: 256, or 512 (4-byte keys, values
                                  staged after keys, 16 keys per lane); 0 = default (512 from 256 buckets) */
-    int32_t reserved[9];
+    int32_t split_reorder;    /* how the WG split writes a tile (the reference's ReorderType,
+                                 opencl/types.h:43-45): EDPP_REORDER_VARIED (the default), _NONE or
+                                 _FIXED; 0 = default                                                  */
+    int32_t reserved[8];
 } edpp_tuning;
+
+enum edpp_reorder {
+    EDPP_REORDER_DEFAULT = 0,
+    EDPP_REORDER_NONE    = 1,  /* each element straight from registers to its output position (NO_REORDER) */
+    EDPP_REORDER_FIXED   = 2,  /* per-bucket LDS staging of 16 elements, written whenever full (FIXED_REORDER) */
+    EDPP_REORDER_VARIED  = 3   /* each bucket's run of a sub-tile staged whole in LDS (VARIED_REORDER)       */
+};
 
 /* Applies to every later call on ctx; NULL restores the defaults. */
 int edpp_set_tuning(edpp_ctx* ctx, const edpp_tuning* t);

@@ -119,10 +119,12 @@ static void test_splits(int len, int buckets) {
     edpp_gen_split_keys_i32(edpp::context(), d_keys, d_vals, len, buckets, EDPP_KEYS_ZIPF, 1234);
     std::vector<int> keys = host(d_keys, len);
     const unsigned mask = buckets - 1;
-    for (int algo = 0; algo < 3; ++algo) {
+    // algo 0 WI, 1..3 WG with VARIED / NO / FIXED reorder, 4 single
+    for (int algo = 0; algo < 5; ++algo) {
+        const edpp::ReorderType ro = algo == 2 ? edpp::NO_REORDER : algo == 3 ? edpp::FIXED_REORDER
+                                                                              : edpp::VARIED_REORDER;
         double ms = algo == 0 ? edpp::WI_split(d_keys, d_ko, d_start, len, buckets, edpp::KVS_SOA, d_vals, d_vo)
-                  : algo == 1 ? edpp::WG_split(d_keys, d_ko, d_start, len, buckets, edpp::VARIED_REORDER,
-                                               edpp::KVS_SOA, d_vals, d_vo)
+                  : algo <= 3 ? edpp::WG_split(d_keys, d_ko, d_start, len, buckets, ro, edpp::KVS_SOA, d_vals, d_vo)
                               : edpp::single_split(d_keys, d_ko, len, buckets, false, edpp::KO);
         CHECK(ms >= 0, "split algo %d returned %f", algo, ms);
         std::vector<int> ko = host(d_ko, len), vo = host(d_vo, len), st = host(d_start, buckets);
@@ -132,13 +134,13 @@ static void test_splits(int len, int buckets) {
             in_sum += keys[i] & mask;
             out_sum += ko[i] & mask;
             if (i && (unsigned)(ko[i] & mask) < (unsigned)(ko[i - 1] & mask)) ok = false;
-            if (algo < 2) {
+            if (algo < 4) {
                 if (ko[i] != keys[vo[i]]) ok = false;                                    // keys follow values
                 if (i && (ko[i] & mask) == (ko[i - 1] & mask) && vo[i] <= vo[i - 1]) ok = false;  // stable
             }
         }
         CHECK(ok && in_sum == out_sum, "split algo %d len %d buckets %d", algo, len, buckets);
-        if (algo < 2) {
+        if (algo < 4) {
             std::vector<int> cnt(buckets, 0);
             for (int i = 0; i < len; ++i) cnt[keys[i] & mask]++;
             int acc = 0;

@@ -146,7 +146,7 @@ def test_scan_full_size_properties(torch_gpu, ctx):
 
 # ---------------------------------------------------------------- local (matrix) scans
 @pytest.mark.parametrize("matrix", [0, 1, 2, 3])
-@pytest.mark.parametrize("network", [0, 1, 2, 3])
+@pytest.mark.parametrize("network", [0, 1, 2, 3, 4, 5, 6, 7, 8])
 @pytest.mark.parametrize("tile,local", [(1, 512), (11, 512), (49, 512), (64, 64), (16, 1024), (3, 128), (2, 512),
                                         (64, 512), (38, 1024)])
 def test_scan_local(torch_gpu, ctx, orc, matrix, network, tile, local):
@@ -315,6 +315,59 @@ def test_split_i32(torch_gpu, ctx, orc, strategy, B, n):
         kout2 = empty(torch, n, torch.int32)
         ctx.split(dk, kout2, B, n=n, strategy=strategy)
         assert np.array_equal(host(kout2), ko)
+
+
+@pytest.mark.parametrize("reorder", [1, 2, 3])  # REORDER_NONE, _FIXED, _VARIED
+@pytest.mark.parametrize("ipt,mult", [(8, 16), (16, 16), (16, 64), (8, 256)])
+@pytest.mark.parametrize("B", [1, 2, 16, 32, 256, 4096])
+def test_split_reorder_variants(torch_gpu, ctx, orc, reorder, ipt, mult, B):
+    # the WG split's three ways of writing a tile (the reference's ReorderType):
+    # tiles of one and of many sub-tiles, so the fixed variant's staging
+    # carries elements across sub-tiles and flushes them at the tile end
+    torch = torch_gpu
+    ctx.set_tuning(split_ipt=ipt, split_tile_mult=mult, split_reorder=reorder)
+    try:
+        for n, dist in ((1, 0), (4096 * 7 + 19, 1), (300_007, 0), (1 << 20, 1)):
+            keys, vals = orc.gen_split_keys(n, B, dist, 77 + n + B)
+            ko, vo, start = orc.split(keys, B, vals)
+            dk, dvv = dev(torch, keys), dev(torch, vals)
+            kout = empty(torch, n, torch.int32)
+            vout = empty(torch, n, torch.int32)
+            st = torch.full((B,), -1, dtype=torch.int32, device="cuda")
+            ctx.split(dk, kout, B, dvv, vout, st, n, 1)
+            assert np.array_equal(host(kout), ko), (reorder, ipt, mult, B, n)
+            assert np.array_equal(host(vout), vo), (reorder, ipt, mult, B, n)
+            assert np.array_equal(host(st), start), (reorder, ipt, mult, B, n)
+            kout2 = empty(torch, n, torch.int32)
+            ctx.split(dk, kout2, B, n=n, strategy=1)  # key-only
+            assert np.array_equal(host(kout2), ko), (reorder, B, n)
+            k64, _ = orc.gen_split_keys(n, B, dist, 91 + n + B, wide=True)
+            k64o, v64o, _ = orc.split(k64, B, vals)
+            d64o = torch.empty(n, dtype=torch.int64, device="cuda")
+            ctx.split(dev(torch, k64), d64o, B, dvv, vout, None, n, 1, wide=True)
+            assert np.array_equal(host(d64o), k64o) and np.array_equal(host(vout), v64o), (reorder, B, n, "wide")
+    finally:
+        ctx.set_tuning()
+
+
+@pytest.mark.parametrize("reorder", [1, 2])
+def test_split_reorder_full_size(torch_gpu, ctx, reorder):
+    # 2^28 pairs through the none and fixed variants, against the default one
+    torch = torch_gpu
+    n = 1 << 28
+    keys = empty(torch, n, torch.int32)
+    vals = empty(torch, n, torch.int32)
+    ctx.gen_split_keys(keys, vals, 256, 0, 4343)
+    ref_k, ref_v = empty(torch, n, torch.int32), empty(torch, n, torch.int32)
+    ctx.split(keys, ref_k, 256, vals, ref_v, None, n, 1)
+    ctx.set_tuning(split_reorder=reorder)
+    try:
+        ko, vo = empty(torch, n, torch.int32), empty(torch, n, torch.int32)
+        ctx.split(keys, ko, 256, vals, vo, None, n, 1)
+        assert torch.equal(ko, ref_k) and torch.equal(vo, ref_v)
+    finally:
+        ctx.set_tuning()
+    del keys, vals, ref_k, ref_v, ko, vo
 
 
 @pytest.mark.parametrize("ipt,mult,block", [(8, 16, 0), (8, 64, 0), (16, 16, 0), (16, 64, 0), (16, 256, 0),
