@@ -97,21 +97,50 @@ __device__ __forceinline__ void chunk_window(const uint32_t (&pos)[U][4], uint32
 
 // out[i] = in[idx[i]] for idx[i] in [from, from+span).  The host clamps
 // from+span to n, so an index outside [0,n) is skipped instead of read.
+// Chunks of the first pass (kGsBlock*UF index vectors each) that its window
+// path finished, from done[]: skip[u] for the workgroup's u-th slice of
+// kGsBlock index vectors, the same in the whole workgroup.  Returns whether
+// every slice is skipped.  A later pass may run with more vectors per lane
+// (U > UF) than the first.
+// ufs = log2 of the first pass's vectors per lane, which divides U; nchunk =
+// the first pass's chunks (flags).  Shifts only: the test runs in every
+// workgroup of every later pass.
+template <int U>
+__device__ __forceinline__ bool later_skips(const uint8_t* __restrict__ done, int64_t ngrp, int ufs, uint32_t nchunk,
+                                            bool (&skip)[U]) {
+    // the workgroup's slices belong to the U >> ufs consecutive first-pass
+    // chunks from blockIdx.x * (U >> ufs), one flag each
+    const int per = U >> ufs;
+    const uint32_t c0 = blockIdx.x * (uint32_t)per;
+    uint32_t mask = 0;
+    for (int f = 0; f < per; ++f)
+        if (c0 + f < nchunk && done[c0 + f]) mask |= 1u << f;
+    bool all = true;
+    #pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int64_t slice = (int64_t)blockIdx.x * U + u;  // its first vector: slice * kGsBlock
+        skip[u] = slice * kGsBlock >= ngrp || ((mask >> (u >> ufs)) & 1u) != 0;
+        all = all && skip[u];
+    }
+    return all;
+}
+
 template <class T, int U, bool RNT>
 __global__ __launch_bounds__(kGsBlock) void k_gather(const T* __restrict__ in, T* __restrict__ out,
                                                      const int32_t* __restrict__ idx, int64_t ngrp, uint32_t from,
                                                      uint32_t span, uint32_t nall, uint8_t* __restrict__ done,
-                                                     int phase) {
+                                                     int phase, int ufs, uint32_t nchunk) {
     __shared__ T win[kGsWin];
     __shared__ uint32_t wlo[kGsBlock / kWave], whi[kGsBlock / kWave];
-    if (phase == kGsLater && done[blockIdx.x]) return;  // the same in the whole workgroup
+    bool skip[U] = {};
+    if (phase == kGsLater && later_skips<U>(done, ngrp, ufs, nchunk, skip)) return;
     const int64_t g0 = (int64_t)blockIdx.x * kGsBlock * U + threadIdx.x;
     uint32_t pos[U][4];
     #pragma unroll
     for (int u = 0; u < U; ++u) {
         int64_t g = g0 + (int64_t)u * kGsBlock;
         v4u32 x = {nall, nall, nall, nall};  // out of range
-        if (g < ngrp) x = ld_stream(reinterpret_cast<const v4u32*>(idx) + g);
+        if (g < ngrp && !skip[u]) x = ld_stream(reinterpret_cast<const v4u32*>(idx) + g);
         #pragma unroll
         for (int k = 0; k < 4; ++k) pos[u][k] = x[k];
     }
@@ -164,11 +193,12 @@ template <class T, int U, bool RNT>
 __global__ __launch_bounds__(kGsBlock) void k_scatter(const T* __restrict__ in, T* __restrict__ out,
                                                       const int32_t* __restrict__ idx, int64_t ngrp, uint32_t from,
                                                       uint32_t span, uint32_t nall, uint8_t* __restrict__ done,
-                                                      int phase) {
+                                                      int phase, int ufs, uint32_t nchunk) {
     __shared__ T win[kGsWin];
     __shared__ uint32_t cov[kGsWin / 32];  // positions of the window this chunk writes
     __shared__ uint32_t wlo[kGsBlock / kWave], whi[kGsBlock / kWave];
-    if (phase == kGsLater && done[blockIdx.x]) return;  // the same in the whole workgroup
+    bool skip[U] = {};
+    if (phase == kGsLater && later_skips<U>(done, ngrp, ufs, nchunk, skip)) return;
     for (int i = threadIdx.x; i < kGsWin / 32; i += kGsBlock) cov[i] = 0;
     const int64_t g0 = (int64_t)blockIdx.x * kGsBlock * U + threadIdx.x;
     uint32_t pos[U][4];
@@ -176,7 +206,7 @@ __global__ __launch_bounds__(kGsBlock) void k_scatter(const T* __restrict__ in, 
     #pragma unroll
     for (int u = 0; u < U; ++u) {
         int64_t g = g0 + (int64_t)u * kGsBlock;
-        if (g < ngrp) {
+        if (g < ngrp && !skip[u]) {
             v4u32 x = ld_stream(reinterpret_cast<const v4u32*>(idx) + g);
             #pragma unroll
             for (int k = 0; k < 4; ++k) pos[u][k] = x[k];
@@ -237,6 +267,8 @@ struct GsPass {
     uint32_t from, span, nall;
     uint8_t* done;  // one flag per workgroup chunk, written by the first of several passes
     int phase;
+    int ufs;          // log2 of the first pass's index vectors per lane (one done flag per kGsBlock << ufs)
+    uint32_t nchunk;  // done flags (workgroups of the first pass)
 };
 
 template <class T, int U, bool RNT>
@@ -245,10 +277,10 @@ static void launch_gs_kernel(edpp_ctx* c, const T* in, T* out, const int32_t* id
     const unsigned grid = (unsigned)ceil_div(ngrp, (int64_t)kGsBlock * U);
     if (scatter)
         hipLaunchKernelGGL((k_scatter<T, U, RNT>), dim3(grid), dim3(kGsBlock), 0, c->stream, in, out, idx, ngrp,
-                           q.from, q.span, q.nall, q.done, q.phase);
+                           q.from, q.span, q.nall, q.done, q.phase, q.ufs, q.nchunk);
     else
         hipLaunchKernelGGL((k_gather<T, U, RNT>), dim3(grid), dim3(kGsBlock), 0, c->stream, in, out, idx, ngrp,
-                           q.from, q.span, q.nall, q.done, q.phase);
+                           q.from, q.span, q.nall, q.done, q.phase, q.ufs, q.nchunk);
 }
 
 template <class T, bool RNT>
@@ -287,12 +319,20 @@ int launch_gather(edpp_ctx* c, const T* in, T* out, const int32_t* idx, int64_t 
         q.nall = (uint32_t)n;
         q.done = done;
         q.phase = passes == 1 ? kGsOnly : (p == 0 ? kGsFirst : kGsLater);
+        const int uf = c->tune.gs_unroll;  // 1, 2, 4 or 8
+        q.ufs = uf == 1 ? 0 : uf == 2 ? 1 : uf == 4 ? 2 : 3;
+        q.nchunk = (uint32_t)ceil_div(ngrp, (int64_t)kGsBlock * uf);
         const uint32_t from = q.from, span = q.span;
+        // scatter passes after the first skip the window test, so for 4-byte
+        // values they take 8 index vectors per lane: half the workgroups
+        // re-reading the index and values (2^28, random, 4 passes: 4.51
+        // against 4.95 ms; gathers measured no steady gain)
+        const int u = (q.phase == kGsLater && scatter && sizeof(T) == 4) ? std::max(8, uf) : uf;
         if (ngrp) {
             if (c->tune.gs_random_nt > 0)
-                dispatch_unroll<T, true>(c, c->tune.gs_unroll, in, out, idx, ngrp, q, scatter);
+                dispatch_unroll<T, true>(c, u, in, out, idx, ngrp, q, scatter);
             else
-                dispatch_unroll<T, false>(c, c->tune.gs_unroll, in, out, idx, ngrp, q, scatter);
+                dispatch_unroll<T, false>(c, u, in, out, idx, ngrp, q, scatter);
         }
         if (ngrp * 4 < n) {
             int64_t rest = n - ngrp * 4;
