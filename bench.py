@@ -59,7 +59,7 @@ def parse():
     p.add_argument("--buckets", type=int, default=256, help="bucket count of --profile split")
     p.add_argument("--profile", default=None,
                    choices=["stream", "gather", "scatter", "gather_local", "scatter_local", "scan", "split", "sort",
-                            "join", "batched", "mlp"],
+                            "join", "batched", "mlp", "probe"],
                    help="only run one workload at full size (for rocprofv3), plus one STREAM copy")
     return p.parse_args()
 
@@ -313,6 +313,13 @@ def profile_only(args, ctx, edpp, torch):
             ctx.scan(a[:m], b[:m], m, edpp.SCAN_AUTO)
         ctx.set_model(None)
         mlp.close()
+    elif args.profile == "probe":
+        # random 4-byte reads, then writes, over 1 GiB / 128 MiB / 2 MiB tables
+        tot = torch.zeros(1, dtype=torch.int64, device="cuda")
+        for lg in (28, 25, 19):
+            for write in (False, True):
+                for _ in range(reps):
+                    ctx.access_random(b if write else a, n, 0xA11CE, write, tot, table_n=1 << lg)
     elif args.profile == "batched":
         del a, b
         batched(ctx, edpp, torch)

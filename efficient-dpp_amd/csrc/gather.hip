@@ -132,15 +132,18 @@ __global__ __launch_bounds__(kGsBlock) void k_gather(const T* __restrict__ in, T
                                                      int phase, int ufs, uint32_t nchunk) {
     __shared__ T win[kGsWin];
     __shared__ uint32_t wlo[kGsBlock / kWave], whi[kGsBlock / kWave];
-    bool skip[U] = {};
-    if (phase == kGsLater && later_skips<U>(done, ngrp, ufs, nchunk, skip)) return;
+    // a gather's later passes use the first pass's vectors per lane: one flag
+    // per workgroup
+    (void)ufs;
+    (void)nchunk;
+    if (phase == kGsLater && done[blockIdx.x]) return;  // the same in the whole workgroup
     const int64_t g0 = (int64_t)blockIdx.x * kGsBlock * U + threadIdx.x;
     uint32_t pos[U][4];
     #pragma unroll
     for (int u = 0; u < U; ++u) {
         int64_t g = g0 + (int64_t)u * kGsBlock;
         v4u32 x = {nall, nall, nall, nall};  // out of range
-        if (g < ngrp && !skip[u]) x = ld_stream(reinterpret_cast<const v4u32*>(idx) + g);
+        if (g < ngrp) x = ld_stream(reinterpret_cast<const v4u32*>(idx) + g);
         #pragma unroll
         for (int k = 0; k < 4; ++k) pos[u][k] = x[k];
     }
