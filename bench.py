@@ -804,9 +804,9 @@ def batched(ctx, edpp, torch):
     tune_t0 = time.time()
     tune = b.autotune(reps=3)
     tune_s = time.time() - tune_t0
-    shape, occ = b.get_shape()
+    shape, occ, order = b.get_shape()
     t_tuned = med(lambda: b.run(timed=True, graph=True), 5)
-    fixed = (0, 4)  # the default shape and occupancy the launch uses without tuning
+    fixed = (0, 4, 0)  # the default shape, occupancy and queue order the launch uses without tuning
     # cold: a 1 GiB write evicts the 256 MiB Infinity Cache before each replay
     scratch = torch.empty(1 << 28, dtype=torch.int32, device="cuda")
 
@@ -833,10 +833,11 @@ def batched(ctx, edpp, torch):
     out = {"jobs": len(plan), "elements": elems, "length_scale": scale, "tiles": b.info()["tiles"],
            "batched_ms": round(t_run, 4), "graph_ms": round(t_graph, 4), "graph_cold_ms": round(t_graph_cold, 4),
            "autotuned_graph_ms": round(t_tuned, 4),
-           "autotuned": {"shape": shape, "wgs_per_cu": occ, "tile_shape": edpp.batch_shapes()[shape],
-                         "search_s": round(tune_s, 2),
+           "autotuned": {"shape": shape, "wgs_per_cu": occ, "queue_order": ("phases", "jobs")[order],
+                         "tile_shape": edpp.batch_shapes()[shape], "search_s": round(tune_s, 2),
                          "fixed_default_ms": round(tune.get(fixed, -1.0), 4),
-                         "all_ms": {f"{k[0]}x{k[1]}": round(v, 4) for k, v in tune.items()}},
+                         "all_ms": {f"{('phases', 'jobs')[k[2]]}_s{k[0]}x{k[1]}": round(v, 4)
+                                    for k, v in tune.items()}},
            "plain_per_call_ms": round(t_plain, 3),
            "speedup": round(t_plain / best, 2), "speedup_fixed_default_shape": round(t_plain / t_graph, 2),
            "batched_gbs": round(nbytes / (best * 1e-3) / 1e9, 1),

@@ -101,7 +101,9 @@ _SIGS["edpp_batch_run_plain"] = [c_vp, ctypes.POINTER(Job), c_i32, c_dp]
 _SIGS["edpp_batch_num_shapes"] = []
 _SIGS["edpp_batch_shape_info"] = [c_i32] + [ctypes.POINTER(c_i32)] * 5
 _SIGS["edpp_batch_set_shape"] = [c_vp, c_i32]
-_SIGS["edpp_batch_get_shape"] = [c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32)]
+_SIGS["edpp_batch_set_order"] = [c_vp, c_i32]
+BATCH_ORDER_PHASES, BATCH_ORDER_JOBS = 0, 1
+_SIGS["edpp_batch_get_shape"] = [c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32), ctypes.POINTER(c_i32)]
 _SIGS["edpp_batch_autotune"] = [c_vp, c_i32, ctypes.POINTER(ctypes.c_double)]
 BATCH_OCCUPANCIES = (2, 4, 8)
 
@@ -462,19 +464,23 @@ class Batch:
     def set_shape(self, shape):
         self.ctx._check(self.ctx._L.edpp_batch_set_shape(self.h, int(shape)))
 
+    def set_order(self, order):
+        self.ctx._check(self.ctx._L.edpp_batch_set_order(self.h, int(order)))
+
     def get_shape(self):
-        """(shape index, workgroups per compute unit) currently set."""
-        s, o = c_i32(), c_i32()
-        self.ctx._check(self.ctx._L.edpp_batch_get_shape(self.h, ctypes.byref(s), ctypes.byref(o)))
-        return s.value, o.value
+        """(shape index, workgroups per compute unit, queue order) currently set."""
+        s, o, q = c_i32(), c_i32(), c_i32()
+        self.ctx._check(self.ctx._L.edpp_batch_get_shape(self.h, ctypes.byref(s), ctypes.byref(o), ctypes.byref(q)))
+        return s.value, o.value, q.value
 
     def autotune(self, reps=3):
-        """Times every shape x occupancy (median of reps); leaves the fastest
-        set and returns {(shape, wgs_per_cu): ms}."""
+        """Times every queue order x shape x occupancy (median of reps); leaves
+        the fastest set and returns {(shape, wgs_per_cu, order): ms}."""
         k = self.ctx._L.edpp_batch_num_shapes()
-        out = (ctypes.c_double * (k * len(BATCH_OCCUPANCIES)))()
+        no = len(BATCH_OCCUPANCIES)
+        out = (ctypes.c_double * (2 * k * no))()
         self.ctx._check(self.ctx._L.edpp_batch_autotune(self.h, int(reps), out))
-        return {(s, o): out[s * len(BATCH_OCCUPANCIES) + i] for s in range(k)
+        return {(s, o, q): out[(q * k + s) * no + i] for q in range(2) for s in range(k)
                 for i, o in enumerate(BATCH_OCCUPANCIES)}
 
     def info(self):

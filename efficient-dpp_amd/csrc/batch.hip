@@ -364,6 +364,7 @@ struct edpp_batch {
     std::vector<edpp_job> jobs;
     int shape = kDefaultShape;
     int wgs_per_cu = 4;
+    bool interleave = false;  // queue order: split scatters among the other tiles, or after them
     // the plan of the current shape
     DJob* d_jobs = nullptr;
     Seg* d_segs = nullptr;
@@ -485,10 +486,17 @@ static int plan(edpp_batch* b) {
         if (dj[j].kind == EDPP_JOB_SPLIT) { add(j, kPhaseHist, dj[j].tiles); hist += dj[j].tiles; }
     for (int32_t j = 0; j < njobs; ++j)
         if (dj[j].kind == EDPP_JOB_SPLIT) { add(j, kPhaseHistScan, dj[j].hscan_tiles); hscan += dj[j].hscan_tiles; }
-    for (int32_t j = 0; j < njobs; ++j)
-        if (dj[j].kind != EDPP_JOB_SPLIT) add(j, kPhaseMain, dj[j].tiles);
-    for (int32_t j = 0; j < njobs; ++j)
-        if (dj[j].kind == EDPP_JOB_SPLIT) add(j, kPhaseScatter, dj[j].tiles);
+    if (b->interleave) {
+        // split scatters in job order among the scan and gather tiles, so
+        // the streaming tiles overlap the gathers' line-fetch-bound ones
+        for (int32_t j = 0; j < njobs; ++j)
+            add(j, dj[j].kind == EDPP_JOB_SPLIT ? kPhaseScatter : kPhaseMain, dj[j].tiles);
+    } else {
+        for (int32_t j = 0; j < njobs; ++j)
+            if (dj[j].kind != EDPP_JOB_SPLIT) add(j, kPhaseMain, dj[j].tiles);
+        for (int32_t j = 0; j < njobs; ++j)
+            if (dj[j].kind == EDPP_JOB_SPLIT) add(j, kPhaseScatter, dj[j].tiles);
+    }
     if (tk >= (1ll << 32)) return fail(c, EDPP_EUNSUP, "batch of %lld tiles is too large", (long long)tk);
     b->total = tk;
     b->hist_tiles = hist;
@@ -568,10 +576,22 @@ int edpp_batch_set_shape(edpp_batch* b, int32_t shape) {
     return plan(b);
 }
 
-int edpp_batch_get_shape(edpp_batch* b, int32_t* shape, int32_t* wgs_per_cu) {
+int edpp_batch_set_order(edpp_batch* b, int32_t order) {
+    if (!b) return EDPP_EINVAL;
+    if (order != EDPP_BATCH_ORDER_PHASES && order != EDPP_BATCH_ORDER_JOBS)
+        return fail(b->ctx, EDPP_EINVAL, "batch order %d unknown", order);
+    const bool il = order == EDPP_BATCH_ORDER_JOBS;
+    if (il == b->interleave && b->state) return EDPP_OK;
+    EDPP_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));  // a run may still use the old plan
+    b->interleave = il;
+    return plan(b);
+}
+
+int edpp_batch_get_shape(edpp_batch* b, int32_t* shape, int32_t* wgs_per_cu, int32_t* order) {
     if (!b) return EDPP_EINVAL;
     if (shape) *shape = b->shape;
     if (wgs_per_cu) *wgs_per_cu = b->wgs_per_cu;
+    if (order) *order = b->interleave ? EDPP_BATCH_ORDER_JOBS : EDPP_BATCH_ORDER_PHASES;
     return EDPP_OK;
 }
 
@@ -613,10 +633,10 @@ int edpp_batch_run_graph(edpp_batch* b, double* ms) {
     return timing_end(c, ms);
 }
 
-// Exhaustive autotune of the batch's launch: every tile shape at every
-// occupancy of kOccupancies, one warm-up run and `reps` timed runs each, the
-// median kept; the fastest pair is left set.  Each run recomputes every job's
-// output from its inputs, so no job may write its own input.
+// Exhaustive autotune of the batch's launch: both queue orders, every tile
+// shape, every occupancy of kOccupancies, one warm-up run and `reps` timed
+// runs each, the median kept; the fastest is left set.  Each run recomputes
+// every job's output from its inputs, so no job may write its own input.
 int edpp_batch_autotune(edpp_batch* b, int32_t reps, double* times) {
     if (!b || reps < 1) return b ? fail(b->ctx, EDPP_EINVAL, "batch autotune: reps < 1") : EDPP_EINVAL;
     edpp_ctx* c = b->ctx;
@@ -626,28 +646,34 @@ int edpp_batch_autotune(edpp_batch* b, int32_t reps, double* times) {
             return fail(c, EDPP_EINVAL, "batch autotune: job %zu works in place, repeated runs would change it", j);
     }
     double best = -1.0;
-    int best_shape = b->shape, best_occ = b->wgs_per_cu;
+    int best_shape = b->shape, best_occ = b->wgs_per_cu, best_order = b->interleave ? 1 : 0;
     std::vector<double> ts((size_t)reps);
-    for (int s = 0; s < kNumShapes; ++s) {
-        int rc = edpp_batch_set_shape(b, s);
+    for (int ord = 0; ord < 2; ++ord) {
+        int rc = edpp_batch_set_order(b, ord);
         if (rc) return rc;
-        for (int o = 0; o < kNumOcc; ++o) {
-            edpp_batch_set_occupancy(b, kOccupancies[o]);
-            double ms = 0.0;
-            rc = edpp_batch_run(b, &ms);  // warm-up
-            for (int r = 0; r < reps && !rc; ++r) rc = edpp_batch_run(b, &ts[(size_t)r]);
+        for (int s = 0; s < kNumShapes; ++s) {
+            rc = edpp_batch_set_shape(b, s);
             if (rc) return rc;
-            std::sort(ts.begin(), ts.end());
-            const double med = ts[(size_t)reps / 2];
-            if (times) times[s * kNumOcc + o] = med;
-            if (best < 0 || med < best) {
-                best = med;
-                best_shape = s;
-                best_occ = kOccupancies[o];
+            for (int o = 0; o < kNumOcc; ++o) {
+                edpp_batch_set_occupancy(b, kOccupancies[o]);
+                double ms = 0.0;
+                rc = edpp_batch_run(b, &ms);  // warm-up
+                for (int r = 0; r < reps && !rc; ++r) rc = edpp_batch_run(b, &ts[(size_t)r]);
+                if (rc) return rc;
+                std::sort(ts.begin(), ts.end());
+                const double med = ts[(size_t)reps / 2];
+                if (times) times[(ord * kNumShapes + s) * kNumOcc + o] = med;
+                if (best < 0 || med < best) {
+                    best = med;
+                    best_order = ord;
+                    best_shape = s;
+                    best_occ = kOccupancies[o];
+                }
             }
         }
     }
-    int rc = edpp_batch_set_shape(b, best_shape);
+    int rc = edpp_batch_set_order(b, best_order);
+    if (!rc) rc = edpp_batch_set_shape(b, best_shape);
     if (!rc) rc = edpp_batch_set_occupancy(b, best_occ);
     return rc;
 }

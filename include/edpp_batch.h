@@ -63,14 +63,22 @@ int edpp_batch_num_shapes(void);
 int edpp_batch_shape_info(int32_t shape, int32_t* block, int32_t* scan_tile32, int32_t* scan_tile64,
                           int32_t* split_tile, int32_t* gather_tile);
 int edpp_batch_set_shape(edpp_batch* b, int32_t shape);
-int edpp_batch_get_shape(edpp_batch* b, int32_t* shape, int32_t* wgs_per_cu);
+/* Queue order of the launch: every split's scatter tiles after all scan and
+ * gather tiles (EDPP_BATCH_ORDER_PHASES, the default), or each job's main
+ * tiles in job order, split scatters included (EDPP_BATCH_ORDER_JOBS), so
+ * that streaming tiles run beside random-gather tiles.  Re-plans the queue;
+ * results do not depend on it. */
+enum edpp_batch_order { EDPP_BATCH_ORDER_PHASES = 0, EDPP_BATCH_ORDER_JOBS = 1 };
+int edpp_batch_set_order(edpp_batch* b, int32_t order);
+int edpp_batch_get_shape(edpp_batch* b, int32_t* shape, int32_t* wgs_per_cu, int32_t* order);
 /* Exhaustive autotune of one batch, as the reference's launch-parameter
  * search tries every local and grid size of a split and keeps the fastest
- * (opencl/test/test_split.cpp:273-358): every shape at 2, 4 and 8 workgroups
- * per compute unit, one warm-up run and `reps` timed runs each (median); the
- * fastest pair stays set.  times (may be NULL) receives
- * edpp_batch_num_shapes() * 3 medians in ms, shape-major.  Runs the batch
- * repeatedly, so jobs writing their own input are rejected (EDPP_EINVAL). */
+ * (opencl/test/test_split.cpp:273-358): both queue orders, every shape, at
+ * 2, 4 and 8 workgroups per compute unit, one warm-up run and `reps` timed
+ * runs each (median); the fastest setting stays.  times (may be NULL)
+ * receives 2 * edpp_batch_num_shapes() * 3 medians in ms, indexed
+ * [order][shape][occupancy].  Runs the batch repeatedly, so jobs writing
+ * their own input are rejected (EDPP_EINVAL). */
 int edpp_batch_autotune(edpp_batch* b, int32_t reps, double* times);
 /* Queue statistics: total tiles, split histogram tiles, histogram-scan tiles. */
 int edpp_batch_info(edpp_batch* b, int64_t* tiles, int64_t* hist_tiles, int64_t* hscan_tiles);
