@@ -57,6 +57,9 @@ def parse():
     p.add_argument("--passes", type=int, default=0, help="pass count of the default step (0 = library schedule)")
     p.add_argument("--out", default=None, help="also write the JSON line to this file")
     p.add_argument("--buckets", type=int, default=256, help="bucket count of --profile split")
+    p.add_argument("--split-strategy", type=int, default=0,
+                   help="strategy of --profile split (0 auto, 1 WG, 2 WI, 3 single, 4 chained) and, when 1 or 4, "
+                        "the digit passes of --profile sort")
     p.add_argument("--profile", default=None,
                    choices=["stream", "gather", "scatter", "gather_local", "scatter_local", "scan", "split", "sort",
                             "join", "batched", "mlp", "probe"],
@@ -279,12 +282,15 @@ def profile_only(args, ctx, edpp, torch):
         vo = torch.empty(n, dtype=i32, device="cuda")
         ctx.gen_split_keys(a, v, args.buckets, edpp.KEYS_UNIFORM, 0x5917)
         for _ in range(reps):
-            ctx.split(a, b, args.buckets, v, vo, None, n, edpp.SPLIT_AUTO)
+            ctx.split(a, b, args.buckets, v, vo, None, n, args.split_strategy)
     elif args.profile == "sort":
         v = torch.arange(n, dtype=i32, device="cuda")
         vo = torch.empty(n, dtype=i32, device="cuda")
+        if args.split_strategy in (edpp.SPLIT_WG, edpp.SPLIT_CHAINED):
+            ctx.set_tuning(sort_strategy=args.split_strategy)
         for _ in range(reps):
             ctx.sort(a, b, v, vo, n, 0, 32)
+        ctx.set_tuning()
     elif args.profile == "join":
         m = 1 << 24
         ctx.gen_perm(a[:m], 0x1A, edpp.PERM_RANDOM, n=m)

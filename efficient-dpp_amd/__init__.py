@@ -19,7 +19,7 @@ SCAN_AUTO, SCAN_LOOKBACK, SCAN_RTS, SCAN_SSA = 0, 1, 2, 3
 MATRIX_LM, MATRIX_REG, MATRIX_LM_REG, MATRIX_LM_SERIAL = 0, 1, 2, 3
 NET_SERIAL, NET_BLELLOCH, NET_SKLANSKY, NET_WAVE = 0, 1, 2, 3
 NET_SERIAL_MT, NET_BLELLOCH_CF, NET_GLOBAL_BLELLOCH, NET_GLOBAL_SKLANSKY, NET_GLOBAL_WAVE = 4, 5, 6, 7, 8
-SPLIT_AUTO, SPLIT_WG, SPLIT_WI, SPLIT_SINGLE = 0, 1, 2, 3
+SPLIT_AUTO, SPLIT_WG, SPLIT_WI, SPLIT_SINGLE, SPLIT_CHAINED = 0, 1, 2, 3, 4
 STREAM_COPY, STREAM_SCALE, STREAM_ADD, STREAM_TRIAD = 0, 1, 2, 3
 PERM_RANDOM, PERM_BLOCK_LOCAL, PERM_IDENTITY = 0, 1, 2
 KEYS_UNIFORM, KEYS_ZIPF = 0, 1
@@ -72,11 +72,11 @@ class Tuning(ctypes.Structure):
     """edpp_tuning of include/edpp_tune.h (0 / -1 fields mean the default)."""
     _fields_ = [("gs_unroll", c_i32), ("gs_random_nt", c_i32), ("scan_vpt", c_i32), ("split_ipt", c_i32),
                 ("split_tile_mult", c_i32), ("scan_block", c_i32), ("split_block", c_i32),
-                ("split_reorder", c_i32), ("reserved", c_i32 * 8)]
+                ("split_reorder", c_i32), ("sort_strategy", c_i32), ("reserved", c_i32 * 7)]
 
 
 TUNING_FIELDS = ("gs_unroll", "gs_random_nt", "scan_vpt", "split_ipt", "split_tile_mult", "scan_block",
-                 "split_block", "split_reorder")
+                 "split_block", "split_reorder", "sort_strategy")
 REORDER_DEFAULT, REORDER_NONE, REORDER_FIXED, REORDER_VARIED = 0, 1, 2, 3
 
 _SIGS["edpp_set_tuning"] = [c_vp, ctypes.POINTER(Tuning)]

@@ -103,6 +103,35 @@ int scan_state_acquire(edpp_ctx* c, int64_t tiles, uint32_t* epoch_out) {
     return EDPP_OK;
 }
 
+int chain_state_acquire(edpp_ctx* c, int64_t words, int epochs, uint32_t* epoch_out) {
+    if (words > c->chain_cap_words) {
+        const int64_t cap = std::max<int64_t>(words, 1 << 20);
+        if (c->chain_status) {
+            EDPP_HIP(c, hipStreamSynchronize(c->stream));
+            hipFree(c->chain_status);
+            c->chain_status = nullptr;
+            c->chain_cap_words = 0;
+        }
+        if (hipMalloc(&c->chain_status, cap * sizeof(uint64_t)) != hipSuccess) {
+            c->chain_status = nullptr;
+            return fail(c, EDPP_ENOMEM, "chained-split state of %lld words", (long long)cap);
+        }
+        int rc = zero_fill(c, c->chain_status, cap * sizeof(uint64_t));
+        if (rc) return rc;
+        c->chain_cap_words = cap;
+        c->chain_epoch = 1;
+    }
+    if (c->chain_epoch + (uint32_t)epochs >= (1u << 30) - 1) {
+        // 30-bit epochs of its own (the scan's alternate its ticket counters)
+        int rc = zero_fill(c, c->chain_status, c->chain_cap_words * sizeof(uint64_t));
+        if (rc) return rc;
+        c->chain_epoch = 1;
+    }
+    *epoch_out = c->chain_epoch;
+    c->chain_epoch += (uint32_t)epochs;
+    return EDPP_OK;
+}
+
 int timing_begin(edpp_ctx* c, double* ms) {
     if (ms) EDPP_HIP(c, hipEventRecord(c->ev_start, c->stream));
     return EDPP_OK;
@@ -253,6 +282,7 @@ int edpp_ctx_destroy(edpp_ctx* c) {
     hipFree(c->ws_outer);
     hipFree(c->scan_status);
     hipFree(c->scan_status64);
+    hipFree(c->chain_status);
     hipFree(c->tickets);
     hipFree(c->d_err);
     if (c->ev_start) hipEventDestroy(c->ev_start);
@@ -281,7 +311,8 @@ int edpp_set_tuning(edpp_ctx* c, const edpp_tuning* t) {
         v.split_tile_mult < 16 || v.split_tile_mult > 1024 ||
         (v.scan_block != 256 && v.scan_block != 512 && v.scan_block != 1024) ||
         (v.split_block != 0 && v.split_block != 256 && v.split_block != 512) ||
-        v.split_reorder < EDPP_REORDER_DEFAULT || v.split_reorder > EDPP_REORDER_VARIED)
+        v.split_reorder < EDPP_REORDER_DEFAULT || v.split_reorder > EDPP_REORDER_VARIED ||
+        (v.sort_strategy != 0 && v.sort_strategy != EDPP_SPLIT_WG && v.sort_strategy != EDPP_SPLIT_CHAINED))
         return fail(c, EDPP_EINVAL, "tuning value out of range");
     c->tune = v;
     return EDPP_OK;
@@ -452,7 +483,7 @@ static int split_common(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, 
         return fail(c, EDPP_EINVAL, "values: both or neither of vals_in/vals_out");  // splitImpl.cpp:29-34
     if (n > 0 && (const void*)kin == (const void*)kout) return fail(c, EDPP_EINVAL, "keys in and out must differ");
     if (n > 0 && vin && (const void*)vin == (const void*)vout) return fail(c, EDPP_EINVAL, "vals in and out must differ");
-    if (strategy < EDPP_SPLIT_AUTO || strategy > EDPP_SPLIT_SINGLE)
+    if (strategy < EDPP_SPLIT_AUTO || strategy > EDPP_SPLIT_CHAINED)
         return fail(c, EDPP_EINVAL, "unknown split strategy %d", strategy);
     int rc = timing_begin(c, ms);
     if (!rc) {

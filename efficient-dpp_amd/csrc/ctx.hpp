@@ -31,6 +31,11 @@ struct edpp_ctx {
     uint32_t* tickets = nullptr;        // [2]
     int32_t* d_err = nullptr;           // device error word written by watchdogs
     uint32_t epoch = 1;                 // 30-bit, 0 never used
+    // chained split: per-tile, per-bucket status words tagged like the scan's,
+    // with epochs of their own
+    uint64_t* chain_status = nullptr;
+    int64_t chain_cap_words = 0;
+    uint32_t chain_epoch = 1;
     uint32_t spin_limit = 1u << 22;     // look-back watchdog (polls before giving up, seconds)
 
     // this device's copy of the no-allocation scan's fixed scratch array
@@ -68,6 +73,9 @@ int ws_outer_reserve(edpp_ctx* c, size_t bytes, void** out);
 int zero_fill(edpp_ctx* c, void* p, size_t bytes);  // kernel, graph-capturable
 // look-back state for at least `tiles` tiles; returns the epoch for this call
 int scan_state_acquire(edpp_ctx* c, int64_t tiles, uint32_t* epoch_out);
+// chained-split status of at least `words` words; returns the first of
+// `epochs` consecutive epochs reserved for this call
+int chain_state_acquire(edpp_ctx* c, int64_t words, int epochs, uint32_t* epoch_out);
 
 // timing helpers: begin() records the start event when ms != nullptr,
 // end() records the stop event, synchronises and writes the elapsed ms

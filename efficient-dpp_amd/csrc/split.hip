@@ -19,6 +19,7 @@
 #include <algorithm>
 #include "ctx.hpp"
 #include "split_tile.hpp"
+#include "tile_scan.hpp"
 
 namespace edpp {
 
@@ -148,6 +149,276 @@ __global__ __launch_bounds__(BLOCK, (BLOCK > kSplitBlock ? 2 * BLOCK / (4 * kWav
 __global__ void k_split_start(const int32_t* scanned, int64_t tiles, int B, int32_t* d_start) {
     for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < B; b += gridDim.x * blockDim.x)
         d_start[b] = scanned[(int64_t)b * tiles];
+}
+
+// ------------------------------------------------------------- chained strategy
+// A pass counts its digit per range of tiles (bucket totals of each range,
+// not of each tile); then its scatter finds each tile's bucket offsets by a
+// look-back over predecessor tiles' per-bucket counts instead of a
+// [B][tiles] matrix written by a histogram kernel and scanned.  The tiles are
+// cut into one
+// contiguous range per XCD: workgroup g takes tiles in order from range
+// g % xcds (the XCD it runs on), so neighbouring tiles, whose runs of a bucket
+// share cache lines, are written through one L2, and a look-back never leaves
+// its range, whose base offset per bucket the counts give.  A workgroup whose
+// range is used up takes a tile from another range; since every range hands
+// out its tiles in order and a tile waits only for earlier tiles of its own
+// range, each of which belongs to a running workgroup, every wait ends.
+
+constexpr int kChainMaxDigits = 4;  // 32 key bits in digits of at most 8
+constexpr int kChainMaxRanges = 8;
+constexpr int kChainStride = 256;   // bucket stride of the count and base tables
+constexpr int kChainWindow = 8;     // predecessors one lane reads per round trip
+
+struct ChainRanges {
+    int64_t tiles;
+    int ranges;  // 1 .. kChainMaxRanges, at most tiles
+    // first tile of range r (r == ranges: tiles); ranges differ by at most one tile
+    __host__ __device__ int64_t first(int r) const {
+        const int64_t per = tiles / ranges, rem = tiles % ranges;
+        return (int64_t)r * per + std::min<int64_t>(r, rem);
+    }
+};
+
+struct ChainDigits {
+    int count;
+    int lg[kChainMaxDigits];
+    int shift[kChainMaxDigits];
+    uint32_t flip[kChainMaxDigits];
+};
+
+// counts[(d*ranges + r)*kChainStride + b] += number of keys of range r whose
+// digit d is b (the kernel counts up to kChainMaxDigits digits per read).  Workgroup j of range r counts the j-th of wpr contiguous
+// parts of the range, keys loaded once for every digit, per-wave LDS rows
+// [waves][digits][kChainStride]; one atomic per nonzero count at the end.
+template <class K, int IPT>
+__global__ __launch_bounds__(kSplitBlock) void k_chain_hist(const K* __restrict__ kin, int64_t n, int64_t tile,
+                                                            ChainRanges cr, int wpr, ChainDigits dg,
+                                                            uint32_t* __restrict__ counts) {
+    extern __shared__ uint32_t lds[];
+    constexpr int S = kSplitBlock * IPT;
+    const int r = blockIdx.x / wpr, j = blockIdx.x % wpr;
+    const int D = dg.count;
+    const int64_t r_lo = cr.first(r) * tile;
+    const int64_t r_hi = std::min<int64_t>(cr.first(r + 1) * tile, n);
+    const int64_t len = std::max<int64_t>(r_hi - r_lo, 0);
+    const int64_t part = ((len + wpr - 1) / wpr + S - 1) / S * S;
+    const int64_t lo = r_lo + (int64_t)j * part;
+    const int64_t hi = std::min<int64_t>(lo + part, r_hi);
+    for (int i = threadIdx.x; i < kSplitWaves * D * kChainStride; i += kSplitBlock) lds[i] = 0;
+    __syncthreads();
+    const int w = threadIdx.x / kWave;
+    for (int64_t s = lo; s < hi; s += S) {
+        const int64_t e0 = s + (int64_t)w * kWave * IPT + lane_id();
+        K key[IPT];
+        #pragma unroll
+        for (int q = 0; q < IPT; ++q) {
+            const int64_t e = e0 + (int64_t)q * kWave;
+            key[q] = e < hi ? kin[e] : K(0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        for (int d = 0; d < D; ++d) {
+            uint32_t* row = lds + (w * D + d) * kChainStride;
+            const int lg = dg.lg[d];
+            switch (lg <= kRegCountBits ? reg_count_words(lg) : 0) {
+                case 1: count_keys<K, IPT, 1>(key, e0, hi, lg, dg.shift[d], dg.flip[d], row); break;
+                case 2: count_keys<K, IPT, 2>(key, e0, hi, lg, dg.shift[d], dg.flip[d], row); break;
+                case 4: count_keys<K, IPT, 4>(key, e0, hi, lg, dg.shift[d], dg.flip[d], row); break;
+                default: count_keys<K, IPT, 0>(key, e0, hi, lg, dg.shift[d], dg.flip[d], row);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < D * kChainStride; i += kSplitBlock) {
+        const int d = i / kChainStride, b = i % kChainStride;
+        if (b < (1 << dg.lg[d])) {
+            uint32_t a = 0;
+            for (int q = 0; q < kSplitWaves; ++q) a += lds[(q * D + d) * kChainStride + b];
+            if (a) fetch_add(counts + (d * cr.ranges + r) * kChainStride + b, a);
+        }
+    }
+}
+
+// base[(d*ranges + r)*kChainStride + b] = output position of the first key of
+// range r in bucket b of digit d: the keys of lower buckets, plus those of
+// bucket b in earlier ranges.  One workgroup per digit; d_start (digit 0)
+// receives the bucket starts.
+__global__ __launch_bounds__(kSplitBlock) void k_chain_bases(const uint32_t* __restrict__ counts, int ranges,
+                                                             ChainDigits dg, uint32_t* __restrict__ base,
+                                                             int32_t* __restrict__ d_start) {
+    __shared__ uint32_t tot[kChainStride], off[kChainStride], wtmp[kSplitWaves];
+    const int d = blockIdx.x, b = threadIdx.x;
+    const int B = 1 << dg.lg[d];
+    uint32_t c[kChainMaxRanges];
+    uint32_t t = 0;
+    #pragma unroll
+    for (int r = 0; r < kChainMaxRanges; ++r) {
+        c[r] = (r < ranges && b < B) ? counts[(d * ranges + r) * kChainStride + b] : 0u;
+        t += c[r];
+    }
+    tot[b] = t;
+    __syncthreads();
+    scan_buckets<kSplitBlock>(tot, off, B, wtmp);
+    __syncthreads();
+    if (b < B) {
+        uint32_t x = off[b];
+        #pragma unroll
+        for (int r = 0; r < kChainMaxRanges; ++r) {
+            if (r < ranges) base[(d * ranges + r) * kChainStride + b] = x;
+            x += c[r];
+        }
+        if (d_start && d == 0) d_start[b] = (int32_t)off[b];
+    }
+}
+
+struct ChainPass {
+    LookbackState32 st;      // status word of tile t, bucket b at t*B + b
+    const uint32_t* base;    // [ranges][kChainStride] of this digit
+    uint32_t* tickets;       // [ranges], zero before the pass
+    ChainRanges cr;
+    int xcds;
+    uint32_t spin_limit;
+    int32_t* err;
+};
+
+// The chained scatter's part of place_and_store.  publish(): thread b < B
+// publishes its bucket's count of the tile (as the inclusive prefix for a
+// range's first tile, else as an aggregate).  resolve(): L consecutive lanes
+// per bucket (L = BLOCK/B, at most a wave) walk back over the range's
+// earlier tiles, L*kChainWindow of them per round trip, nearest first,
+// adding aggregates until they meet an inclusive prefix (a round whose
+// nearest words up to that prefix include an unpublished one is read again);
+// the segment's first lane publishes the tile's inclusive prefix and sets
+// run[b].  A barrier follows resolve(), since run[b] is read by thread b.
+struct ChainHook {
+    static constexpr bool kSync = true;
+    LookbackState32 st;
+    const uint32_t* base;
+    uint32_t spin_limit;
+    int32_t* err;
+    int64_t tile, first;
+    int range, B, L;
+    const uint32_t* sub;
+    uint32_t* run;
+    __device__ __forceinline__ void publish() const {
+        const int b = threadIdx.x;
+        if (b < B) st.publish(tile * B + b, tile == first ? kFlagPrefix : kFlagAggregate, sub[b]);
+    }
+    __device__ __forceinline__ void resolve() const {
+        const int x = threadIdx.x;
+        if (x >= B * L) return;  // whole waves
+        const int b = x / L, j = x % L;
+        const int sbase = lane_id() & ~(L - 1);
+        const uint64_t smask = L == kWave ? ~0ull : (1ull << L) - 1ull;
+        uint32_t excl = 0;
+        if (tile > first) {
+            int64_t cur = tile - 1;  // nearest predecessor not yet added
+            uint32_t spins = 0;
+            while (true) {
+                uint32_t f[kChainWindow], v[kChainWindow];
+                #pragma unroll
+                for (int q = 0; q < kChainWindow; ++q) {
+                    const int64_t p = cur - j - (int64_t)L * q;
+                    if (p >= first) {
+                        st.read(p * B + b, f[q], v[q]);
+                    } else {
+                        f[q] = kFlagPrefix;  // before the range: its base, added below
+                        v[q] = 0;
+                    }
+                }
+                bool done = false, stalled = false;
+                #pragma unroll
+                for (int q = 0; q < kChainWindow; ++q) {
+                    if (done || stalled) continue;
+                    const uint64_t pm = (ballot(f[q] == kFlagPrefix) >> sbase) & smask;
+                    const uint64_t zm = (ballot(f[q] == kFlagNone) >> sbase) & smask;
+                    const uint64_t upto = pm ? (pm ^ (pm - 1)) : smask;  // lanes up to the nearest prefix
+                    if (zm & upto) {
+                        stalled = true;
+                        continue;
+                    }
+                    uint32_t a = ((upto >> j) & 1ull) ? v[q] : 0u;
+                    for (int o = 1; o < L; o <<= 1) a += (uint32_t)__shfl_xor((int)a, o);
+                    excl += a;
+                    if (pm) done = true;
+                    else cur -= L;
+                }
+                if (done) break;
+                if (stalled) {
+                    if (++spins > spin_limit) {
+                        if (j == 0) st_relaxed(err, (int32_t)EDPP_ETIMEOUT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            if (j == 0) st.publish(tile * B + b, kFlagPrefix, excl + sub[b]);
+        }
+        if (j == 0) run[b] = base[range * kChainStride + b] + excl;
+    }
+};
+
+// One tile (= one sub-tile of S = BLOCK*IPT elements) per workgroup; the
+// grid has one workgroup per tile.  LDS layout and the ranking are those of
+// k_split_scatter (varied reorder).
+template <class K, int IPT, bool KV, bool BALLOT, int BLOCK>
+__global__ __launch_bounds__(BLOCK, (BLOCK > kSplitBlock ? 2 * BLOCK / (4 * kWave) : 1)) void k_chain_scatter(const K* __restrict__ kin, K* __restrict__ kout,
+                                                         const int32_t* __restrict__ vin, int32_t* __restrict__ vout,
+                                                         SplitGeom g, ChainPass cp) {
+    extern __shared__ uint32_t lds[];
+    __shared__ int64_t s_tile;
+    __shared__ int s_range;
+    constexpr int W = BLOCK / kWave;
+    constexpr bool SEP = KV && BLOCK > kSplitBlock;
+    const int B = 1 << g.lg;
+    constexpr int S = BLOCK * IPT;
+    uint64_t* wmask = reinterpret_cast<uint64_t*>(lds);
+    uint32_t* whist = lds + whist_word_offset(B, W);
+    uint32_t* run = whist + W * B;
+    uint32_t* sub = run + B;
+    uint32_t* off = sub + B;
+    uint32_t* wtmp = off + B;
+    K* kbuf = reinterpret_cast<K*>(lds + kbuf_word_offset(B, W));
+    int32_t* vbuf = SEP ? reinterpret_cast<int32_t*>(kbuf) : reinterpret_cast<int32_t*>(kbuf + S);
+    if (threadIdx.x == 0) {
+        const int R = cp.cr.ranges;
+        const int r0 = (int)(blockIdx.x % (unsigned)cp.xcds) % R;
+        int64_t t = -1;
+        int rr = 0;
+        for (int k = 0; k < R && t < 0; ++k) {
+            const int r = r0 + k < R ? r0 + k : r0 + k - R;
+            const int64_t f = cp.cr.first(r), len = cp.cr.first(r + 1) - f;
+            if ((int64_t)ld_relaxed(cp.tickets + r) >= len) continue;
+            const int64_t q = (int64_t)fetch_add(cp.tickets + r, 1u);
+            if (q < len) {
+                t = f + q;
+                rr = r;
+            }
+        }
+        s_tile = t;
+        s_range = rr;
+    }
+    for (int b = threadIdx.x; b < W * B; b += BLOCK) {
+        wmask[b] = 0ull;
+        whist[b] = 0;
+    }
+    __syncthreads();
+    const int64_t tile = s_tile;
+    if (tile < 0) return;  // more workgroups than tiles (never launched so)
+    const int range = s_range;
+    const int64_t tbase = tile * S;
+    const int64_t tend = std::min<int64_t>(tbase + S, g.n);
+    const int w = threadIdx.x / kWave;
+    uint32_t bk[IPT], rk[IPT];
+    K key[IPT];
+    int32_t val[IPT];
+    rank_subtile<K, IPT, BALLOT>(kin, KV ? vin : nullptr, tbase, tend, g.lg, g.shift, whist + w * B, wmask + w * B, bk,
+                                 rk, key, val, g.flip);
+    __syncthreads();
+    const int L = std::min(kWave, std::max(1, BLOCK >> g.lg));
+    const ChainHook hook = {cp.st, cp.base, cp.spin_limit, cp.err, tile, cp.cr.first(range), range, B, L, sub, run};
+    place_and_store<K, IPT, KV, BLOCK, SEP, ChainHook>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk, key, val, B,
+                                                       g.shift, (int)(tend - tbase), kout, vout, g.n, g.flip, hook);
 }
 
 // ------------------------------------------------------------- WI strategy
@@ -491,6 +762,120 @@ int run_pass(edpp_ctx* c, int strategy, const SplitCall<K>& a, int32_t* his) {
     }
 }
 
+// Workspace of the chained strategy's small tables: counts and tickets (one
+// fill clears both), then the bases.
+constexpr size_t kChainTableWords = (size_t)kChainMaxDigits * kChainMaxRanges * kChainStride;
+constexpr size_t kChainSmallBytes =
+    ((2 * kChainTableWords + (size_t)kChainMaxDigits * kChainMaxRanges) * 4 + 255) & ~(size_t)255;
+
+template <class K>
+struct ChainStep {
+    const K* kin;
+    K* kout;
+    const int32_t* vin;
+    int32_t* vout;
+    int lg, shift;
+    uint32_t flip;
+};
+
+template <class K, int IPT, bool BALLOT, int BLOCK>
+int run_chain_pass(edpp_ctx* c, const ChainStep<K>& a, int64_t n, const ChainPass& cp) {
+    SplitGeom g;
+    g.n = n;
+    g.lg = a.lg;
+    g.shift = a.shift;
+    g.tile = (int64_t)BLOCK * IPT;
+    g.tiles = cp.cr.tiles;
+    g.xcds = c->xcds;
+    g.flip = a.flip;
+    const size_t ls = wg_lds_bytes<IPT, K, BLOCK>(1 << a.lg);
+    if (a.vin) {
+        EDPP_HIP(c, hipFuncSetAttribute((const void*)k_chain_scatter<K, IPT, true, BALLOT, BLOCK>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls));
+        hipLaunchKernelGGL((k_chain_scatter<K, IPT, true, BALLOT, BLOCK>), dim3((unsigned)g.tiles), dim3(BLOCK), ls,
+                           c->stream, a.kin, a.kout, a.vin, a.vout, g, cp);
+    } else {
+        EDPP_HIP(c, hipFuncSetAttribute((const void*)k_chain_scatter<K, IPT, false, BALLOT, BLOCK>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls));
+        hipLaunchKernelGGL((k_chain_scatter<K, IPT, false, BALLOT, BLOCK>), dim3((unsigned)g.tiles), dim3(BLOCK), ls,
+                           c->stream, a.kin, a.kout, a.vin, a.vout, g, cp);
+    }
+    EDPP_HIP(c, hipGetLastError());
+    return EDPP_OK;
+}
+
+// Chained split of n elements in `digits` stable LSD passes (each of at most
+// 8 bits; pass p reads steps[p].kin and writes steps[p].kout).  Every pass
+// uses the tile shape the WG scatter picks for the widest digit, so that all
+// passes share one cut into ranges.  Each pass counts its digit per range of
+// its own input (a later pass's ranges hold what the pass before wrote
+// there), turns the counts into range bases and runs the chained scatter.
+// d_start, when given, receives the first pass's bucket starts.  `small`
+// holds kChainSmallBytes of workspace.
+template <class K>
+int launch_chain(edpp_ctx* c, int64_t n, int digits, const ChainStep<K>* steps, int32_t* d_start, uint32_t* small) {
+    int maxlg = 0;
+    for (int p = 0; p < digits; ++p) maxlg = std::max(maxlg, steps[p].lg);
+    const int ipt = wg_ipt<K>(c, maxlg), block = wg_block<K>(c, maxlg);
+    const int64_t S = (int64_t)block * ipt;
+    ChainRanges cr;
+    cr.tiles = ceil_div(n, S);
+    cr.ranges = (int)std::min<int64_t>(std::min<int64_t>(std::max(c->xcds, 1), kChainMaxRanges), cr.tiles);
+    uint32_t* counts = small;
+    uint32_t* tickets = small + kChainTableWords;
+    uint32_t* base = tickets + (size_t)kChainMaxDigits * kChainMaxRanges;
+    int rc = zero_fill(c, counts, (kChainTableWords + (size_t)kChainMaxDigits * kChainMaxRanges) * 4);
+    if (rc) return rc;
+    uint32_t epoch0;
+    rc = chain_state_acquire(c, cr.tiles << maxlg, digits, &epoch0);
+    if (rc) return rc;
+    // about four counting workgroups per CU, none of them without keys
+    constexpr int64_t SH = (int64_t)kSplitBlock * 16;
+    const int64_t per_range = ceil_div(cr.tiles, (int64_t)cr.ranges) * S;
+    const int wpr = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(4 * (int64_t)std::max(c->cus, 1), cr.ranges),
+                                                                 ceil_div(per_range, SH)));
+    const size_t lh = (size_t)kSplitWaves * kChainStride * 4;
+    for (int p = 0; p < digits && rc == EDPP_OK; ++p) {
+        ChainDigits dg = {};
+        dg.count = 1;
+        dg.lg[0] = steps[p].lg;
+        dg.shift[0] = steps[p].shift;
+        dg.flip[0] = steps[p].flip;
+        uint32_t* cnt_p = counts + (size_t)p * cr.ranges * kChainStride;
+        uint32_t* base_p = base + (size_t)p * cr.ranges * kChainStride;
+        hipLaunchKernelGGL((k_chain_hist<K, 16>), dim3((unsigned)(cr.ranges * wpr)), dim3(kSplitBlock), lh, c->stream,
+                           steps[p].kin, n, S, cr, wpr, dg, cnt_p);
+        hipLaunchKernelGGL(k_chain_bases, dim3(1), dim3(kSplitBlock), 0, c->stream, (const uint32_t*)cnt_p, cr.ranges,
+                           dg, base_p, p == 0 ? d_start : nullptr);
+        EDPP_HIP(c, hipGetLastError());
+        ChainPass cp;
+        cp.st.status = c->chain_status;
+        cp.st.epoch = epoch0 + (uint32_t)p;
+        cp.base = base_p;
+        cp.tickets = tickets + (size_t)p * cr.ranges;
+        cp.cr = cr;
+        cp.xcds = std::max(c->xcds, 1);
+        cp.spin_limit = c->spin_limit;
+        cp.err = c->d_err;
+        const bool bal = steps[p].lg <= kBallotMatchBits;
+        if (ipt == 16) {
+            if constexpr (sizeof(K) == 4) {
+                if (block == 512) {
+                    rc = bal ? run_chain_pass<K, 16, true, 512>(c, steps[p], n, cp)
+                             : run_chain_pass<K, 16, false, 512>(c, steps[p], n, cp);
+                    continue;
+                }
+            }
+            rc = bal ? run_chain_pass<K, 16, true, kSplitBlock>(c, steps[p], n, cp)
+                     : run_chain_pass<K, 16, false, kSplitBlock>(c, steps[p], n, cp);
+        } else {
+            rc = bal ? run_chain_pass<K, 8, true, kSplitBlock>(c, steps[p], n, cp)
+                     : run_chain_pass<K, 8, false, kSplitBlock>(c, steps[p], n, cp);
+        }
+    }
+    return rc;
+}
+
 }  // namespace
 
 // Bucket counts above kMaxSplitBitsPerPass bits are split as two stable LSD
@@ -518,10 +903,35 @@ int launch_split(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t
         c->tune = saved;
         return rc;
     }
-    if (strategy < EDPP_SPLIT_WG || strategy > EDPP_SPLIT_SINGLE)
+    if (strategy < EDPP_SPLIT_WG || strategy > EDPP_SPLIT_CHAINED)
         return fail(c, EDPP_EINVAL, "unknown split strategy %d", strategy);
     const bool two = strategy != EDPP_SPLIT_SINGLE && lg > kMaxSplitBitsPerPass;
     const int lg1 = two ? lg / 2 : lg, lg2 = lg - lg1;
+    if (strategy == EDPP_SPLIT_CHAINED) {
+        // both digits of a two-pass split are counted by the one up-front read
+        const size_t kbytes = two ? (((size_t)n * sizeof(K) + 255) & ~(size_t)255) : 0;
+        const size_t vbytes = (two && vin) ? (((size_t)n * 4 + 255) & ~(size_t)255) : 0;
+        void* w;
+        int rc = ws_reserve(c, kChainSmallBytes + kbytes + vbytes, &w);
+        if (rc) return rc;
+        char* base = static_cast<char*>(w);
+        uint32_t* small = reinterpret_cast<uint32_t*>(base);
+        if (!two) {
+            const ChainStep<K> st = {kin, kout, vin, vout, lg, 0, 0u};
+            return launch_chain<K>(c, n, 1, &st, d_start, small);
+        }
+        K* tk = reinterpret_cast<K*>(base + kChainSmallBytes);
+        int32_t* tv = vin ? reinterpret_cast<int32_t*>(base + kChainSmallBytes + kbytes) : nullptr;
+        const ChainStep<K> st[2] = {{kin, tk, vin, tv, lg1, 0, 0u}, {tk, kout, tv, vout, lg2, lg1, 0u}};
+        rc = launch_chain<K>(c, n, 2, st, nullptr, small);
+        if (rc) return rc;
+        if (d_start) {
+            hipLaunchKernelGGL((k_split_start_search<K>), dim3((unsigned)ceil_div(buckets, 256)), dim3(256), 0,
+                               c->stream, kout, n, lg, d_start);
+            EDPP_HIP(c, hipGetLastError());
+        }
+        return EDPP_OK;
+    }
     int64_t hlen = his_len<K>(c, strategy, n, lg1);
     if (two) hlen = std::max(hlen, his_len<K>(c, strategy, n, lg2));
     if (hlen >= (1ll << 31)) return fail(c, EDPP_EUNSUP, "histogram of %lld entries too large", (long long)hlen);
@@ -589,7 +999,8 @@ int launch_sort_u32(edpp_ctx* c, const uint32_t* kin, uint32_t* kout, const int3
     const bool first_out = (passes % 2) == 1;  // the first pass writes the output
     const bool need_a = passes > 1 || (alias && first_out);
     const bool need_b = alias && first_out && passes > 1;
-    const size_t hbytes = ((size_t)hlen * 4 + 255) & ~(size_t)255;
+    const bool chained = c->tune.sort_strategy == EDPP_SPLIT_CHAINED;
+    const size_t hbytes = chained ? kChainSmallBytes : ((size_t)hlen * 4 + 255) & ~(size_t)255;
     const size_t kb = ((size_t)n * 4 + 255) & ~(size_t)255;
     const size_t vb = vin ? kb : 0;
     const size_t copies = (need_a ? 1 : 0) + (need_b ? 1 : 0);
@@ -618,7 +1029,19 @@ int launch_sort_u32(edpp_ctx* c, const uint32_t* kin, uint32_t* kout, const int3
     }
     const uint32_t* sk = kin;
     const int32_t* sv = vin;
-    for (int p = 0; p < passes && rc == EDPP_OK; ++p) {
+    if (chained) {
+        // every digit counted in one read of the keys, then one chained scatter per digit
+        ChainStep<uint32_t> st[kChainMaxDigits];
+        for (int p = 0; p < passes; ++p) {
+            const int lo = begin_bit + p * per, hi = std::min(end_bit, lo + per);
+            st[p] = {sk, const_cast<uint32_t*>(dk[p]), sv, const_cast<int32_t*>(dv[p]), hi - lo, lo,
+                     (is_signed && hi == 32) ? (1u << (hi - lo - 1)) : 0u};
+            sk = dk[p];
+            sv = dv[p];
+        }
+        rc = launch_chain<uint32_t>(c, n, passes, st, nullptr, reinterpret_cast<uint32_t*>(his));
+    }
+    for (int p = 0; p < passes && rc == EDPP_OK && !chained; ++p) {
         const int lo = begin_bit + p * per, hi = std::min(end_bit, lo + per), lg = hi - lo;
         const uint32_t flip = (is_signed && hi == 32) ? (1u << (lg - 1)) : 0u;
         SplitCall<uint32_t> a = {sk, const_cast<uint32_t*>(dk[p]), sv, const_cast<int32_t*>(dv[p]), nullptr, n, lg, lo,

@@ -160,6 +160,116 @@ __device__ __forceinline__ void count_subtile(const K* __restrict__ kin, int64_t
 constexpr int kRegCountBits = 4;
 __host__ __device__ constexpr int reg_count_words(int lg) { return lg <= 2 ? 1 : (1 << (lg - 2)); }
 
+// Counts one lane's IPT keys of a sub-tile, item j being element
+// e0 + j*64 (elements at or past n ignored), into the wave's row: row[b] +=
+// the wave's count of bucket b.  REGW > 0 selects the register counters
+// (B <= 4*REGW), REGW == 0 LDS atomics.  Wave-uniform control flow.
+template <class K, int IPT, int REGW>
+__device__ __forceinline__ void count_keys(const K (&key)[IPT], int64_t e0, int64_t n, int lg, int shift,
+                                           uint32_t flip, uint32_t* row) {
+    static_assert(IPT <= 255, "8-bit per-lane counters");
+    const int lane = lane_id();
+    const int B = 1 << lg;
+    const uint32_t mask = (uint32_t)B - 1u;
+    if constexpr (REGW >= 2) {
+        // 8 or 16 buckets: 4-bit counters, bucket b at bits [4b, 4b+4) of one
+        // 64-bit word per item parity (at most IPT/2 <= 15 items each), so a
+        // key costs a shift and an add instead of a compare-and-select per
+        // packed word (a wave64 instruction holds a SIMD for 4 cycles)
+        static_assert(IPT / 2 <= 15, "4-bit counters");
+        uint64_t acc0 = 0, acc1 = 0;
+        #pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            const int64_t e = e0 + (int64_t)j * kWave;
+            const uint32_t b = bucket_of(key[j], shift, mask, flip);
+            const uint64_t inc = e < n ? 1ull << (4 * b) : 0ull;
+            if (j & 1) acc1 += inc;
+            else acc0 += inc;
+        }
+        constexpr uint64_t kNib = 0x0F0F0F0F0F0F0F0Full;
+        const uint64_t ev = (acc0 & kNib) + (acc1 & kNib);                // byte k: bucket 2k
+        const uint64_t od = ((acc0 >> 4) & kNib) + ((acc1 >> 4) & kNib);  // byte k: bucket 2k+1
+        #pragma unroll
+        for (int h = 0; h < REGW / 2; ++h) {  // buckets 8h .. 8h+7
+            const uint32_t we = (uint32_t)(ev >> (32 * h)), wo = (uint32_t)(od >> (32 * h));
+            // two 16-bit fields per sum: at most 64*IPT per bucket
+            const uint32_t e04 = wave_sum(we & 0x00FF00FFu);         // buckets 8h, 8h+4
+            const uint32_t e26 = wave_sum((we >> 8) & 0x00FF00FFu);  // buckets 8h+2, 8h+6
+            const uint32_t o15 = wave_sum(wo & 0x00FF00FFu);         // buckets 8h+1, 8h+5
+            const uint32_t o37 = wave_sum((wo >> 8) & 0x00FF00FFu);  // buckets 8h+3, 8h+7
+            if (lane == 0) {  // the row belongs to this wave alone
+                const int b0 = 8 * h;
+                row[b0 + 0] += e04 & 0xFFFFu;
+                row[b0 + 4] += e04 >> 16;
+                row[b0 + 2] += e26 & 0xFFFFu;
+                row[b0 + 6] += e26 >> 16;
+                row[b0 + 1] += o15 & 0xFFFFu;
+                row[b0 + 5] += o15 >> 16;
+                row[b0 + 3] += o37 & 0xFFFFu;
+                row[b0 + 7] += o37 >> 16;
+            }
+        }
+    } else if constexpr (REGW > 0) {
+        uint32_t packed[REGW];  // bucket b: word b/4, bits [8(b%4), 8(b%4)+8)
+        #pragma unroll
+        for (int q = 0; q < REGW; ++q) packed[q] = 0;
+        #pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            const int64_t e = e0 + (int64_t)j * kWave;
+            const uint32_t b = bucket_of(key[j], shift, mask, flip);
+            const uint32_t inc = e < n ? 1u << (8 * (b & 3u)) : 0u;
+            #pragma unroll
+            for (int q = 0; q < REGW; ++q) packed[q] += (b >> 2) == (uint32_t)q ? inc : 0u;
+        }
+        #pragma unroll
+        for (int q = 0; q < REGW; ++q) {
+            // two 16-bit fields per word: at most 64*IPT per bucket
+            const uint32_t even = wave_sum(packed[q] & 0x00FF00FFu);        // buckets 4q, 4q+2
+            const uint32_t odd = wave_sum((packed[q] >> 8) & 0x00FF00FFu);  // buckets 4q+1, 4q+3
+            if (lane == 0) {  // the row belongs to this wave alone
+                const int b0 = 4 * q;
+                if (b0 < B) row[b0] += even & 0xFFFFu;
+                if (b0 + 1 < B) row[b0 + 1] += odd & 0xFFFFu;
+                if (b0 + 2 < B) row[b0 + 2] += even >> 16;
+                if (b0 + 3 < B) row[b0 + 3] += odd >> 16;
+            }
+        }
+    } else {
+        // skewed keys (an eighth of the wave or more in the first lane's
+        // bucket on the first or the middle item) take the grouped path for
+        // the whole sub-tile; the test is wave-uniform
+        uint32_t bf0, bf1;
+        const uint64_t mf0 = first_bucket_group(bucket_of(key[0], shift, mask, flip), e0 < n, &bf0);
+        const uint64_t mf1 =
+            first_bucket_group(bucket_of(key[IPT / 2], shift, mask, flip), e0 + (int64_t)(IPT / 2) * kWave < n, &bf1);
+        if (__popcll(mf0) >= kWave / 8 || __popcll(mf1) >= kWave / 8) {
+            #pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int64_t e = e0 + (int64_t)j * kWave;
+                const bool ok = e < n;
+                const uint32_t b = bucket_of(key[j], shift, mask, flip);
+                uint32_t bf;
+                const uint64_t mf = first_bucket_group(b, ok, &bf);
+                if (ok && b == bf) {
+                    if (count_below(mf) == 0)
+                        __hip_atomic_fetch_add(row + bf, (uint32_t)__popcll(mf), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+                } else if (ok) {
+                    __hip_atomic_fetch_add(row + b, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                }
+            }
+        } else {
+            #pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int64_t e = e0 + (int64_t)j * kWave;
+                if (e < n)
+                    __hip_atomic_fetch_add(row + bucket_of(key[j], shift, mask, flip), 1u, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WAVEFRONT);
+            }
+        }
+    }
+}
+
 // Counts TPW consecutive sub-tiles of S = 256*IPT elements, sub-tile t
 // starting at sbase + t*S (elements at or past n ignored), into the rows of
 // tile t/spt: whist + ((t/spt)*waves + w)*B.  Element layout as in
@@ -170,11 +280,9 @@ template <class K, int IPT, int TPW, int REGW>
 __device__ __forceinline__ void count_tiles(const K* __restrict__ kin, int64_t sbase, int64_t n, int lg, int shift,
                                             int spt, uint32_t* whist, uint32_t flip = 0) {
     constexpr int S = kSplitBlock * IPT;
-    static_assert(IPT <= 255, "8-bit per-lane counters");
     const int lane = lane_id();
     const int w = threadIdx.x / kWave;
     const int B = 1 << lg;
-    const uint32_t mask = (uint32_t)B - 1u;
     const int64_t wbase = sbase + (int64_t)w * kWave * IPT + lane;
     K key[TPW][IPT];
     #pragma unroll
@@ -186,107 +294,9 @@ __device__ __forceinline__ void count_tiles(const K* __restrict__ kin, int64_t s
         }
     __builtin_amdgcn_sched_barrier(0);
     #pragma unroll
-    for (int t = 0; t < TPW; ++t) {
-        uint32_t* row = whist + ((t / spt) * kSplitWaves + w) * B;
-        if constexpr (REGW >= 2) {
-            // 8 or 16 buckets: 4-bit counters, bucket b at bits [4b, 4b+4) of one
-            // 64-bit word per item parity (at most IPT/2 <= 15 items each), so a
-            // key costs a shift and an add instead of a compare-and-select per
-            // packed word (a wave64 instruction holds a SIMD for 4 cycles)
-            static_assert(IPT / 2 <= 15, "4-bit counters");
-            uint64_t acc0 = 0, acc1 = 0;
-            #pragma unroll
-            for (int j = 0; j < IPT; ++j) {
-                const int64_t e = wbase + (int64_t)t * S + (int64_t)j * kWave;
-                const uint32_t b = bucket_of(key[t][j], shift, mask, flip);
-                const uint64_t inc = e < n ? 1ull << (4 * b) : 0ull;
-                if (j & 1) acc1 += inc;
-                else acc0 += inc;
-            }
-            constexpr uint64_t kNib = 0x0F0F0F0F0F0F0F0Full;
-            const uint64_t ev = (acc0 & kNib) + (acc1 & kNib);                // byte k: bucket 2k
-            const uint64_t od = ((acc0 >> 4) & kNib) + ((acc1 >> 4) & kNib);  // byte k: bucket 2k+1
-            #pragma unroll
-            for (int h = 0; h < REGW / 2; ++h) {  // buckets 8h .. 8h+7
-                const uint32_t we = (uint32_t)(ev >> (32 * h)), wo = (uint32_t)(od >> (32 * h));
-                // two 16-bit fields per sum: at most 64*IPT per bucket
-                const uint32_t e04 = wave_sum(we & 0x00FF00FFu);         // buckets 8h, 8h+4
-                const uint32_t e26 = wave_sum((we >> 8) & 0x00FF00FFu);  // buckets 8h+2, 8h+6
-                const uint32_t o15 = wave_sum(wo & 0x00FF00FFu);         // buckets 8h+1, 8h+5
-                const uint32_t o37 = wave_sum((wo >> 8) & 0x00FF00FFu);  // buckets 8h+3, 8h+7
-                if (lane == 0) {  // the row belongs to this wave alone
-                    const int b0 = 8 * h;
-                    row[b0 + 0] += e04 & 0xFFFFu;
-                    row[b0 + 4] += e04 >> 16;
-                    row[b0 + 2] += e26 & 0xFFFFu;
-                    row[b0 + 6] += e26 >> 16;
-                    row[b0 + 1] += o15 & 0xFFFFu;
-                    row[b0 + 5] += o15 >> 16;
-                    row[b0 + 3] += o37 & 0xFFFFu;
-                    row[b0 + 7] += o37 >> 16;
-                }
-            }
-        } else if constexpr (REGW > 0) {
-            uint32_t packed[REGW];  // bucket b: word b/4, bits [8(b%4), 8(b%4)+8)
-            #pragma unroll
-            for (int q = 0; q < REGW; ++q) packed[q] = 0;
-            #pragma unroll
-            for (int j = 0; j < IPT; ++j) {
-                const int64_t e = wbase + (int64_t)t * S + (int64_t)j * kWave;
-                const uint32_t b = bucket_of(key[t][j], shift, mask, flip);
-                const uint32_t inc = e < n ? 1u << (8 * (b & 3u)) : 0u;
-                #pragma unroll
-                for (int q = 0; q < REGW; ++q) packed[q] += (b >> 2) == (uint32_t)q ? inc : 0u;
-            }
-            #pragma unroll
-            for (int q = 0; q < REGW; ++q) {
-                // two 16-bit fields per word: at most 64*IPT per bucket
-                const uint32_t even = wave_sum(packed[q] & 0x00FF00FFu);        // buckets 4q, 4q+2
-                const uint32_t odd = wave_sum((packed[q] >> 8) & 0x00FF00FFu);  // buckets 4q+1, 4q+3
-                if (lane == 0) {  // the row belongs to this wave alone
-                    const int b0 = 4 * q;
-                    if (b0 < B) row[b0] += even & 0xFFFFu;
-                    if (b0 + 1 < B) row[b0 + 1] += odd & 0xFFFFu;
-                    if (b0 + 2 < B) row[b0 + 2] += even >> 16;
-                    if (b0 + 3 < B) row[b0 + 3] += odd >> 16;
-                }
-            }
-        } else {
-            // skewed keys (an eighth of the wave or more in the first lane's
-            // bucket on the first or the middle item) take the grouped path
-            // for the whole sub-tile; the test is wave-uniform
-            uint32_t bf0, bf1;
-            const uint64_t mf0 = first_bucket_group(bucket_of(key[t][0], shift, mask, flip),
-                                                    wbase + (int64_t)t * S < n, &bf0);
-            const uint64_t mf1 = first_bucket_group(bucket_of(key[t][IPT / 2], shift, mask, flip),
-                                                    wbase + (int64_t)t * S + (int64_t)(IPT / 2) * kWave < n, &bf1);
-            if (__popcll(mf0) >= kWave / 8 || __popcll(mf1) >= kWave / 8) {
-                #pragma unroll
-                for (int j = 0; j < IPT; ++j) {
-                    const int64_t e = wbase + (int64_t)t * S + (int64_t)j * kWave;
-                    const bool ok = e < n;
-                    const uint32_t b = bucket_of(key[t][j], shift, mask, flip);
-                    uint32_t bf;
-                    const uint64_t mf = first_bucket_group(b, ok, &bf);
-                    if (ok && b == bf) {
-                        if (count_below(mf) == 0)
-                            __hip_atomic_fetch_add(row + bf, (uint32_t)__popcll(mf), __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
-                    } else if (ok) {
-                        __hip_atomic_fetch_add(row + b, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-                    }
-                }
-            } else {
-                #pragma unroll
-                for (int j = 0; j < IPT; ++j) {
-                    const int64_t e = wbase + (int64_t)t * S + (int64_t)j * kWave;
-                    if (e < n)
-                        __hip_atomic_fetch_add(row + bucket_of(key[t][j], shift, mask, flip), 1u, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WAVEFRONT);
-                }
-            }
-        }
-    }
+    for (int t = 0; t < TPW; ++t)
+        count_keys<K, IPT, REGW>(key[t], wbase + (int64_t)t * S, n, lg, shift, flip,
+                                 whist + ((t / spt) * kSplitWaves + w) * B);
 }
 
 // Stable ranks of the wave's chunk: rk[j] = whist_w[b] (count of earlier keys
@@ -375,19 +385,30 @@ __device__ __forceinline__ void scan_buckets(const uint32_t* cnt, uint32_t* off,
 // the sub-tile's count of b; sub[] is reused for run[b] - off[b].  cnt is the
 // number of valid elements (S for every sub-tile but the last).
 //
+// hook.publish() runs right after the sub-tile's bucket totals are in sub[],
+// and hook.resolve() right before run[] is read (followed by a barrier when
+// Hook::kSync), on every thread: the chained scatter publishes its counts in
+// the first and sets run[] from its look-back in the second.
+//
 // SEP: the values follow the keys through the same buffer (vbuf == kbuf), so
 // a sub-tile twice as long fits the LDS of one staged with its keys; longer
 // bucket runs leave as fewer partial-line stores.  Otherwise keys and values
 // are staged together and streamed out by a plain loop (each store leaves as
 // soon as its slot is read, which measured faster than one burst).  Ends
 // without a barrier.
-template <class K, int IPT, bool KV, int BLOCK = kSplitBlock, bool SEP = false>
+struct NoPlaceHook {
+    static constexpr bool kSync = false;  // resolve() writes nothing another thread reads
+    __device__ __forceinline__ void publish() const {}
+    __device__ __forceinline__ void resolve() const {}
+};
+
+template <class K, int IPT, bool KV, int BLOCK = kSplitBlock, bool SEP = false, class Hook = NoPlaceHook>
 __device__ __forceinline__ void place_and_store(uint32_t* whist, uint32_t* run, uint32_t* sub, uint32_t* off,
                                                 uint32_t* wtmp, K* kbuf, int32_t* vbuf, const uint32_t (&bk)[IPT],
                                                 const uint32_t (&rk)[IPT], const K (&key)[IPT],
                                                 const int32_t (&val)[IPT], int B, int shift, int cnt,
                                                 K* __restrict__ kout, int32_t* __restrict__ vout, int64_t n,
-                                                uint32_t flip = 0) {
+                                                uint32_t flip = 0, const Hook& hook = Hook()) {
     constexpr int W = BLOCK / kWave;
     constexpr int S = BLOCK * IPT;
     const int w = threadIdx.x / kWave;
@@ -406,6 +427,7 @@ __device__ __forceinline__ void place_and_store(uint32_t* whist, uint32_t* run, 
         sub[b] = a;
     }
     __syncthreads();
+    hook.publish();
     scan_buckets<BLOCK>(sub, off, B, wtmp);
     __syncthreads();
     uint32_t pos[IPT];
@@ -421,6 +443,8 @@ __device__ __forceinline__ void place_and_store(uint32_t* whist, uint32_t* run, 
     }
     // each thread owns its buckets here: nobody reads sub[] or run[] again
     // before the barrier below
+    hook.resolve();
+    if constexpr (Hook::kSync) __syncthreads();
     for (int b = threadIdx.x; b < B; b += BLOCK) {
         const uint32_t c = sub[b];
         sub[b] = run[b] - off[b];

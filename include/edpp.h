@@ -87,7 +87,9 @@ enum edpp_split_strategy {       /* SPLIT_ALGO / ReorderType, opencl/types.h:32-
     EDPP_SPLIT_AUTO     = 0,
     EDPP_SPLIT_WG       = 1,     /* per-tile histogram -> scan of [B][tiles] -> stable scatter, LDS reorder */
     EDPP_SPLIT_WI       = 2,     /* per-wave histograms [B][waves] -> scan -> stable scatter, no reorder     */
-    EDPP_SPLIT_SINGLE   = 3      /* one lane per contiguous chunk (the reference's single_split)            */
+    EDPP_SPLIT_SINGLE   = 3,     /* one lane per contiguous chunk (the reference's single_split)            */
+    EDPP_SPLIT_CHAINED  = 4      /* bucket counts per XCD range of tiles, then a scatter whose tiles chain
+                                    their bucket offsets by a per-bucket look-back (no [B][tiles] matrix) */
 };
 
 enum edpp_stream_op {            /* opencl/kernels/mem_kernel.cl:9-31 */

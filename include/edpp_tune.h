@@ -30,7 +30,10 @@ typedef struct edpp_tuning {
     int32_t split_reorder;    /* how the WG split writes a tile (the reference's ReorderType,
                                  opencl/types.h:43-45): EDPP_REORDER_VARIED (the default), _NONE or
                                  _FIXED; 0 = default                                                  */
-    int32_t reserved[8];
+    int32_t sort_strategy;    /* digit passes of a radix sort: EDPP_SPLIT_WG (per-tile histogram, scan,
+                                 scatter) or EDPP_SPLIT_CHAINED (per-range counts, chained scatter);
+                                 0 = default                                                          */
+    int32_t reserved[7];
 } edpp_tuning;
 
 enum edpp_reorder {

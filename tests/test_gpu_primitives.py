@@ -295,7 +295,7 @@ def test_gather_scatter_full_size_properties(torch_gpu, ctx, kind):
 SPLIT_SIZES = [0, 1, 63, 100, 4096 * 3 + 5, 100_003, 1 << 20]
 
 
-@pytest.mark.parametrize("strategy", [1, 2, 3])
+@pytest.mark.parametrize("strategy", [1, 2, 3, 4])  # WG, WI, SINGLE, CHAINED
 @pytest.mark.parametrize("B", [1, 2, 4, 8, 16, 32, 256, 4096])
 @pytest.mark.parametrize("n", SPLIT_SIZES)
 def test_split_i32(torch_gpu, ctx, orc, strategy, B, n):
@@ -398,7 +398,7 @@ def test_split_wg_shapes(torch_gpu, ctx, orc, ipt, mult, block, B):
         ctx.set_tuning()
 
 
-@pytest.mark.parametrize("strategy", [1, 2])
+@pytest.mark.parametrize("strategy", [1, 2, 4])
 @pytest.mark.parametrize("B", [2, 256, 4096])
 def test_split_i64(torch_gpu, ctx, orc, strategy, B):
     torch = torch_gpu
@@ -412,8 +412,9 @@ def test_split_i64(torch_gpu, ctx, orc, strategy, B):
     assert np.array_equal(host(kout), ko) and np.array_equal(host(vout), vo) and np.array_equal(host(st), start)
 
 
+@pytest.mark.parametrize("strategy", [1, 4])
 @pytest.mark.parametrize("B", [2, 16, 256])
-def test_split_unaligned_input(torch_gpu, ctx, orc, B):
+def test_split_unaligned_input(torch_gpu, ctx, orc, B, strategy):
     # keys and values one word past a 16-byte boundary
     torch = torch_gpu
     n = 100_003
@@ -426,13 +427,14 @@ def test_split_unaligned_input(torch_gpu, ctx, orc, B):
     kout = empty(torch, n, torch.int32)
     vout = empty(torch, n, torch.int32)
     st = empty(torch, B, torch.int32)
-    ctx.split(dk[1:], kout, B, dvv[1:], vout, st, n, 1)
+    ctx.split(dk[1:], kout, B, dvv[1:], vout, st, n, strategy)
     assert np.array_equal(host(kout), ko) and np.array_equal(host(vout), vo) and np.array_equal(host(st), start)
 
 
+@pytest.mark.parametrize("strategy", [1, 4])
 @pytest.mark.parametrize("dist", [0, 1])
 @pytest.mark.parametrize("B", [2, 16, 256, 4096])
-def test_split_full_size_properties(torch_gpu, ctx, B, dist):
+def test_split_full_size_properties(torch_gpu, ctx, B, dist, strategy):
     # 2^28 pairs, uniform or Zipf keys, vals = original index: bucket ids
     # non-decreasing, stable (vals increasing inside each bucket), a
     # permutation, keys follow their values, offsets equal the bucket counts
@@ -444,7 +446,7 @@ def test_split_full_size_properties(torch_gpu, ctx, B, dist):
     kout = empty(torch, n, torch.int32)
     vout = empty(torch, n, torch.int32)
     st = empty(torch, B, torch.int32)
-    ctx.split(keys, kout, B, vals, vout, st, n, 1)
+    ctx.split(keys, kout, B, vals, vout, st, n, strategy)
     b = kout & (B - 1)
     assert bool((b[1:] >= b[:-1]).all())
     same = b[1:] == b[:-1]
@@ -457,11 +459,23 @@ def test_split_full_size_properties(torch_gpu, ctx, B, dist):
 
 
 # ---------------------------------------------------------------- radix sort
+SORT_STRATEGIES = [1, 4]  # per-digit WG splits, or one count of every digit and chained scatters
+
+
+@pytest.fixture
+def sort_ctx(ctx, request):
+    ctx.set_tuning(sort_strategy=request.param)
+    yield ctx
+    ctx.set_tuning()
+
+
+@pytest.mark.parametrize("sort_ctx", SORT_STRATEGIES, indirect=True)
 @pytest.mark.parametrize("signed", [False, True])
 @pytest.mark.parametrize("bits", [(0, 32), (0, 16), (4, 20), (8, 32), (0, 8), (3, 4), (7, 7)])
 @pytest.mark.parametrize("n", [0, 1, 1000, 100_003, (1 << 20) - 1, (1 << 20) + 17, 3_000_001])
-def test_sort(torch_gpu, ctx, orc, signed, bits, n):
+def test_sort(torch_gpu, sort_ctx, orc, signed, bits, n):
     torch = torch_gpu
+    ctx = sort_ctx
     keys = orc.gen_i32(n, 500 + n + bits[0])
     if n > 10:
         keys[: n // 10] = keys[n // 10: 2 * (n // 10)]  # duplicates: stability observable
@@ -483,11 +497,13 @@ def test_sort(torch_gpu, ctx, orc, signed, bits, n):
     assert np.array_equal(host(dk), ko.view(np.int32)) and np.array_equal(host(vin), vo), ("in place", bits, n)
 
 
+@pytest.mark.parametrize("sort_ctx", SORT_STRATEGIES, indirect=True)
 @pytest.mark.parametrize("kind", ["equal", "two", "zipf"])
-def test_sort_skewed_keys(torch_gpu, ctx, orc, kind):
-    # one-sweep passes with every key of a tile in one bucket (equal), two
-    # values (0 and -1: the flipped top digit orders -1 first), Zipf low bytes
+def test_sort_skewed_keys(torch_gpu, sort_ctx, orc, kind):
+    # passes with every key of a tile in one bucket (equal), two values (0
+    # and -1: the flipped top digit orders -1 first), Zipf low bytes
     torch = torch_gpu
+    ctx = sort_ctx
     n = 3_000_001
     if kind == "equal":
         keys = np.full(n, 0x12345678, np.int32)
@@ -505,10 +521,12 @@ def test_sort_skewed_keys(torch_gpu, ctx, orc, kind):
         assert np.array_equal(host(vout), vo), (kind, signed)
 
 
-def test_sort_full_size_properties(torch_gpu, ctx):
+@pytest.mark.parametrize("sort_ctx", SORT_STRATEGIES, indirect=True)
+def test_sort_full_size_properties(torch_gpu, sort_ctx):
     # 2^28 pairs, signed 32-bit keys, vals = original index: keys ascending,
     # ties in input order, a permutation, keys follow their values
     torch = torch_gpu
+    ctx = sort_ctx
     n = 1 << 28
     keys = empty(torch, n, torch.int32)
     ctx.gen_i32(keys, 0x50F7)
