@@ -60,7 +60,7 @@ struct Cand {
 const Cand kCands[K] = {
     {0, "scan look-back 1024x8", EDPP_SCAN_LOOKBACK, 1024, 8},
     {0, "scan look-back 1024x4", EDPP_SCAN_LOOKBACK, 1024, 4},
-    {0, "scan look-back 512x16", EDPP_SCAN_LOOKBACK, 512, 16},
+    {0, "scan look-back 1024x16", EDPP_SCAN_LOOKBACK, 1024, 16},
     {0, "scan reduce-then-scan", EDPP_SCAN_RTS, 0, 0},
     {0, "scan scan-scan-add", EDPP_SCAN_SSA, 0, 0},
     {1, "gather/scatter 1 pass, unroll 4", 1, 4, 0},

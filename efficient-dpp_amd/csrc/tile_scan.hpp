@@ -134,6 +134,42 @@ struct TileScan {
         total = sm.total;
     }
 
+    // After scan(): turns v[j][k] into the element's exclusive prefix inside
+    // the tile, so that off[] is dead and a tile needs VPT*EPV registers
+    // through the look-back instead of VPT*(EPV+1); store_folded() then adds
+    // the tile's prefix.
+    __device__ __forceinline__ void fold() {
+        #pragma unroll
+        for (int j = 0; j < VPT; ++j) {
+            T x = off[j];
+            #pragma unroll
+            for (int k = 0; k < EPV; ++k) {
+                const T a = v[j][k];
+                v[j][k] = x;
+                x += a;
+            }
+        }
+    }
+    __device__ __forceinline__ void store_folded(T* out, int64_t base, int64_t n, T prefix) const {
+        const int t = threadIdx.x;
+        const bool full = base + TILE <= n;
+        #pragma unroll
+        for (int j = 0; j < VPT; ++j) {
+            V o;
+            #pragma unroll
+            for (int k = 0; k < EPV; ++k) o[k] = prefix + v[j][k];
+            const int64_t e0 = base + (int64_t)j * SUB + (int64_t)t * EPV;
+            if (full || e0 + EPV <= n) {
+                st_stream(o, reinterpret_cast<V*>(out + e0));
+            } else {
+                #pragma unroll
+                for (int k = 0; k < EPV; ++k)
+                    if (e0 + k < n) out[e0 + k] = o[k];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+
     __device__ __forceinline__ void store(T* out, int64_t base, int64_t n, T prefix) const {
         const int t = threadIdx.x;
         const bool full = base + TILE <= n;

@@ -93,6 +93,35 @@ def test_scan_i64(torch_gpu, ctx, orc, scheme, n):
     assert int(host(total)[0]) == int(tot)
 
 
+@pytest.mark.parametrize("block,vpt", [(256, 2), (256, 4), (256, 8), (256, 16), (512, 4), (512, 8), (512, 16),
+                                       (1024, 4), (1024, 8), (1024, 16)])
+def test_scan_lookback_shapes(torch_gpu, ctx, orc, block, vpt):
+    # every look-back tile shape (lanes x 16-byte vectors per lane), 32- and
+    # 64-bit, whole and ragged last tiles, in place
+    torch = torch_gpu
+    ctx.set_tuning(scan_block=block, scan_vpt=vpt)
+    try:
+        for n in (1, 1000, (1 << 20) + 3, 3_000_001, 1 << 24):
+            x = orc.gen_i32(n, 300 + n + vpt, -1)
+            ref, tot = orc.scan_exclusive(x)
+            out = empty(torch, n, torch.int32)
+            total = torch.zeros(1, dtype=torch.int32, device="cuda")
+            ctx.scan(dev(torch, x), out, n, scheme=1, total=total)
+            assert np.array_equal(host(out), ref), (block, vpt, n)
+            assert int(host(total)[0]) == int(tot)
+            d = dev(torch, x)
+            ctx.scan(d, d, n, scheme=1)
+            assert np.array_equal(host(d), ref), ("in place", block, vpt, n)
+        for n in (4097, 3_000_001):
+            x = orc.gen_i64(n, 17 + n + vpt)
+            ref, tot = orc.scan_exclusive(x)
+            out = empty(torch, n, torch.int64)
+            ctx.scan(dev(torch, x), out, n, scheme=1, wide=True)
+            assert np.array_equal(host(out), ref), ("i64", block, vpt, n)
+    finally:
+        ctx.set_tuning()
+
+
 @pytest.mark.parametrize("scheme", [1, 2, 3])
 def test_scan_in_place_and_unaligned(torch_gpu, ctx, orc, scheme):
     torch = torch_gpu
