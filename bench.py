@@ -812,7 +812,7 @@ def batched(ctx, edpp, torch):
     tune_s = time.time() - tune_t0
     shape, occ, order = b.get_shape()
     t_tuned = med(lambda: b.run(timed=True, graph=True), 5)
-    fixed = (0, 4, 0)  # the default shape, occupancy and queue order the launch uses without tuning
+    fixed = (0, 4, edpp.BATCH_ORDER_XCD)  # the default shape, occupancy and queue order without tuning
     # cold: a 1 GiB write evicts the 256 MiB Infinity Cache before each replay
     scratch = torch.empty(1 << 28, dtype=torch.int32, device="cuda")
 
@@ -839,10 +839,10 @@ def batched(ctx, edpp, torch):
     out = {"jobs": len(plan), "elements": elems, "length_scale": scale, "tiles": b.info()["tiles"],
            "batched_ms": round(t_run, 4), "graph_ms": round(t_graph, 4), "graph_cold_ms": round(t_graph_cold, 4),
            "autotuned_graph_ms": round(t_tuned, 4),
-           "autotuned": {"shape": shape, "wgs_per_cu": occ, "queue_order": ("phases", "jobs")[order],
+           "autotuned": {"shape": shape, "wgs_per_cu": occ, "queue_order": edpp.BATCH_ORDERS[order],
                          "tile_shape": edpp.batch_shapes()[shape], "search_s": round(tune_s, 2),
                          "fixed_default_ms": round(tune.get(fixed, -1.0), 4),
-                         "all_ms": {f"{('phases', 'jobs')[k[2]]}_s{k[0]}x{k[1]}": round(v, 4)
+                         "all_ms": {f"{edpp.BATCH_ORDERS[k[2]]}_s{k[0]}x{k[1]}": round(v, 4)
                                     for k, v in tune.items()}},
            "plain_per_call_ms": round(t_plain, 3),
            "speedup": round(t_plain / best, 2), "speedup_fixed_default_shape": round(t_plain / t_graph, 2),

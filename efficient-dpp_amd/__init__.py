@@ -102,7 +102,8 @@ _SIGS["edpp_batch_num_shapes"] = []
 _SIGS["edpp_batch_shape_info"] = [c_i32] + [ctypes.POINTER(c_i32)] * 5
 _SIGS["edpp_batch_set_shape"] = [c_vp, c_i32]
 _SIGS["edpp_batch_set_order"] = [c_vp, c_i32]
-BATCH_ORDER_PHASES, BATCH_ORDER_JOBS = 0, 1
+BATCH_ORDER_PHASES, BATCH_ORDER_JOBS, BATCH_ORDER_XCD = 0, 1, 2
+BATCH_ORDERS = ("phases", "jobs", "xcd")
 _SIGS["edpp_batch_get_shape"] = [c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32), ctypes.POINTER(c_i32)]
 _SIGS["edpp_batch_autotune"] = [c_vp, c_i32, ctypes.POINTER(ctypes.c_double)]
 BATCH_OCCUPANCIES = (2, 4, 8)
@@ -478,9 +479,10 @@ class Batch:
         the fastest set and returns {(shape, wgs_per_cu, order): ms}."""
         k = self.ctx._L.edpp_batch_num_shapes()
         no = len(BATCH_OCCUPANCIES)
-        out = (ctypes.c_double * (2 * k * no))()
+        nq = len(BATCH_ORDERS)
+        out = (ctypes.c_double * (nq * k * no))()
         self.ctx._check(self.ctx._L.edpp_batch_autotune(self.h, int(reps), out))
-        return {(s, o, q): out[(q * k + s) * no + i] for q in range(2) for s in range(k)
+        return {(s, o, q): out[(q * k + s) * no + i] for q in range(nq) for s in range(k)
                 for i, o in enumerate(BATCH_OCCUPANCIES)}
 
     def info(self):

@@ -9,8 +9,14 @@
 // job's histogram tiles, a scatter tile for its job's histogram scan, a scan
 // tile (and a histogram-scan tile) for its predecessors in the look-back.
 // Tickets are handed out in order, so whatever a workgroup waits for is held
-// by a workgroup that is already running; every wait ends.  The reference has
-// no counterpart (its primitives are one synchronous call each).
+// by a workgroup that is already running; every wait ends.  In the XCD order
+// the jobs are dealt to one queue per XCD (each queue ordered as above over
+// its own jobs, with its own ticket counter); workgroup g starts on queue
+// g % xcds, the queue of the XCD it runs on, so a job's tiles share one L2,
+// and moves to the next queue when its own is used up.  Every job lies in one
+// queue, so every wait still points to a smaller ticket of a queue that hands
+// out its tickets in order.  The reference has no counterpart (its primitives
+// are one synchronous call each).
 #include <algorithm>
 #include <vector>
 #include "ctx.hpp"
@@ -38,8 +44,20 @@ struct DJob {
 };
 
 struct Seg {
-    int64_t first;  // first ticket
+    int64_t first;  // first ticket within its queue
     int32_t job, phase;
+};
+
+constexpr int kMaxQueues = 8;
+constexpr int kTicketStride = 32;  // 32-bit words between the queues' ticket counters (one 128-byte line each)
+
+// The queues of a launch: queue q holds segments [seg_first[q], seg_first[q]
+// + seg_count[q]) of the segment table and total[q] tickets.
+struct QTable {
+    int64_t total[kMaxQueues];
+    int32_t seg_first[kMaxQueues];
+    int32_t seg_count[kMaxQueues];
+    int32_t nq, xcds;
 };
 
 // Tile shapes of the persistent launch: lanes per workgroup, 16-byte vectors
@@ -61,6 +79,7 @@ constexpr int kNumShapes = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
 constexpr int kDefaultShape = 0;
 constexpr int kOccupancies[] = {2, 4, 8};
 constexpr int kNumOcc = 3;
+constexpr int kNumOrders = EDPP_BATCH_ORDER_XCD + 1;
 
 __host__ __device__ inline int64_t tile_elems(const BatchShape& s, int kind, int wide) {
     if (kind == EDPP_JOB_SCAN) return (int64_t)s.block * (wide ? 2 * s.scan_vpt64 : 4 * s.scan_vpt32);
@@ -284,17 +303,22 @@ __global__ void k_batch_init(uint32_t* state, int64_t words, const DJob* jobs, i
 
 template <int BLOCK, int SV32, int SV64, int IPT, int GU>
 __global__ __launch_bounds__(BLOCK) void k_batch(const DJob* __restrict__ jobs, const Seg* __restrict__ segs,
-                                                 int32_t nseg, int64_t total, uint32_t* ticket, uint32_t spin_limit,
-                                                 int32_t* err) {
+                                                 QTable qt, uint32_t* tickets, uint32_t spin_limit, int32_t* err) {
     extern __shared__ __align__(16) char smem[];
     __shared__ uint32_t s_ticket;
+    int q = qt.nq == 1 ? 0 : (int)(blockIdx.x % (unsigned)qt.xcds) % qt.nq;  // this XCD's queue
+    int spent = 0;  // queues found used up
     while (true) {
-        if (threadIdx.x == 0) s_ticket = fetch_add(ticket, 1u);
+        if (threadIdx.x == 0) s_ticket = fetch_add(tickets + q * kTicketStride, 1u);
         __syncthreads();
         const int64_t tk = s_ticket;
         __syncthreads();  // all lanes hold the ticket before lane 0 may take the next
-        if (tk >= total) break;
-        int lo = 0, hi = nseg - 1;
+        if (tk >= qt.total[q]) {
+            if (++spent >= qt.nq) break;
+            q = q + 1 == qt.nq ? 0 : q + 1;
+            continue;
+        }
+        int lo = qt.seg_first[q], hi = qt.seg_first[q] + qt.seg_count[q] - 1;
         while (lo < hi) {
             int mid = (lo + hi + 1) >> 1;
             if (segs[mid].first <= tk) lo = mid;
@@ -343,7 +367,7 @@ constexpr size_t shape_lds(const BatchShape& s) {
                             1024);
 }
 
-typedef void (*BatchKernel)(const DJob*, const Seg*, int32_t, int64_t, uint32_t*, uint32_t, int32_t*);
+typedef void (*BatchKernel)(const DJob*, const Seg*, QTable, uint32_t*, uint32_t, int32_t*);
 
 template <int I>
 constexpr BatchKernel kernel_of() {
@@ -364,11 +388,11 @@ struct edpp_batch {
     std::vector<edpp_job> jobs;
     int shape = kDefaultShape;
     int wgs_per_cu = 4;
-    bool interleave = false;  // queue order: split scatters among the other tiles, or after them
+    int order = EDPP_BATCH_ORDER_XCD;  // EDPP_BATCH_ORDER_*
     // the plan of the current shape
     DJob* d_jobs = nullptr;
     Seg* d_segs = nullptr;
-    int32_t nseg = 0;
+    QTable qt = {};
     int64_t total = 0, hist_tiles = 0, hscan_tiles = 0;
     uint32_t* state = nullptr;  // zeroed each run: ticket, look-back status, split histogram words
     int64_t state_words = 0;
@@ -403,8 +427,7 @@ static int batch_launch(edpp_batch* b, hipStream_t s) {
         const BatchShape& sh = kShapes[b->shape];
         const int64_t grid = std::min<int64_t>(b->total, (int64_t)c->cus * b->wgs_per_cu);
         hipLaunchKernelGGL(kKernels[b->shape], dim3((unsigned)grid), dim3(sh.block), shape_lds(sh), s,
-                           (const DJob*)b->d_jobs, (const Seg*)b->d_segs, b->nseg, b->total, b->state,
-                           c->spin_limit, c->d_err);
+                           (const DJob*)b->d_jobs, (const Seg*)b->d_segs, b->qt, b->state, c->spin_limit, c->d_err);
     }
     EDPP_HIP(c, hipGetLastError());
     return EDPP_OK;
@@ -440,9 +463,9 @@ static int plan(edpp_batch* b) {
     const int32_t njobs = (int32_t)b->jobs.size();
     drop_plan(b);
     std::vector<DJob> dj(njobs);
-    // state layout in 32-bit words: [0] ticket, then per job its look-back
-    // status words and, for a split, its tagged histogram words
-    int64_t words = 64;  // ticket and padding
+    // state layout in 32-bit words: the queues' ticket counters, then per job
+    // its look-back status words and, for a split, its tagged histogram words
+    int64_t words = (int64_t)kMaxQueues * kTicketStride;
     std::vector<int64_t> soff(njobs, -1);
     for (int32_t j = 0; j < njobs; ++j) {
         const edpp_job& q = b->jobs[j];
@@ -470,9 +493,33 @@ static int plan(edpp_batch* b) {
         }
         words = (words + 3) & ~(int64_t)3;  // 16-byte alignment of every job's state
     }
+    // jobs to queues: one queue, or (XCD order) one per XCD, each job to the
+    // least loaded queue, largest jobs first, by the bytes it moves (a random
+    // gather read counted as 16 bytes)
+    const int nq = b->order == EDPP_BATCH_ORDER_XCD ? std::max(1, std::min(c->xcds, kMaxQueues)) : 1;
+    std::vector<int32_t> qof(njobs, 0);
+    if (nq > 1) {
+        std::vector<int32_t> idx(njobs);
+        std::vector<double> cost(njobs), load(nq, 0.0);
+        for (int32_t j = 0; j < njobs; ++j) {
+            idx[j] = j;
+            const double w = dj[j].wide ? 8.0 : 4.0, n = (double)dj[j].n;
+            cost[j] = dj[j].kind == EDPP_JOB_SCAN ? 2 * w * n : (dj[j].kind == EDPP_JOB_SPLIT ? (3 * w + 8) * n
+                                                                                              : (4 + w + 16) * n);
+        }
+        std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b2) { return cost[a] > cost[b2]; });
+        for (int32_t j : idx) {
+            const int qq = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            qof[j] = qq;
+            load[qq] += cost[j];
+        }
+    }
     // segments in queue order
     std::vector<Seg> segs;
-    int64_t tk = 0, hist = 0, hscan = 0;
+    QTable qt = {};
+    qt.nq = nq;
+    qt.xcds = std::max(1, c->xcds);
+    int64_t tk = 0, total = 0, hist = 0, hscan = 0;
     auto add = [&](int32_t j, int32_t phase, int64_t count) {
         if (count <= 0) return;
         Seg s;
@@ -482,26 +529,37 @@ static int plan(edpp_batch* b) {
         segs.push_back(s);
         tk += count;
     };
-    for (int32_t j = 0; j < njobs; ++j)
-        if (dj[j].kind == EDPP_JOB_SPLIT) { add(j, kPhaseHist, dj[j].tiles); hist += dj[j].tiles; }
-    for (int32_t j = 0; j < njobs; ++j)
-        if (dj[j].kind == EDPP_JOB_SPLIT) { add(j, kPhaseHistScan, dj[j].hscan_tiles); hscan += dj[j].hscan_tiles; }
-    if (b->interleave) {
-        // split scatters in job order among the scan and gather tiles, so
-        // the streaming tiles overlap the gathers' line-fetch-bound ones
+    for (int q = 0; q < nq; ++q) {
+        tk = 0;
+        qt.seg_first[q] = (int32_t)segs.size();
         for (int32_t j = 0; j < njobs; ++j)
-            add(j, dj[j].kind == EDPP_JOB_SPLIT ? kPhaseScatter : kPhaseMain, dj[j].tiles);
-    } else {
+            if (qof[j] == q && dj[j].kind == EDPP_JOB_SPLIT) { add(j, kPhaseHist, dj[j].tiles); hist += dj[j].tiles; }
         for (int32_t j = 0; j < njobs; ++j)
-            if (dj[j].kind != EDPP_JOB_SPLIT) add(j, kPhaseMain, dj[j].tiles);
-        for (int32_t j = 0; j < njobs; ++j)
-            if (dj[j].kind == EDPP_JOB_SPLIT) add(j, kPhaseScatter, dj[j].tiles);
+            if (qof[j] == q && dj[j].kind == EDPP_JOB_SPLIT) {
+                add(j, kPhaseHistScan, dj[j].hscan_tiles);
+                hscan += dj[j].hscan_tiles;
+            }
+        if (b->order != EDPP_BATCH_ORDER_PHASES) {
+            // split scatters in job order among the scan and gather tiles, so
+            // the streaming tiles overlap the gathers' line-fetch-bound ones
+            for (int32_t j = 0; j < njobs; ++j)
+                if (qof[j] == q) add(j, dj[j].kind == EDPP_JOB_SPLIT ? kPhaseScatter : kPhaseMain, dj[j].tiles);
+        } else {
+            for (int32_t j = 0; j < njobs; ++j)
+                if (qof[j] == q && dj[j].kind != EDPP_JOB_SPLIT) add(j, kPhaseMain, dj[j].tiles);
+            for (int32_t j = 0; j < njobs; ++j)
+                if (qof[j] == q && dj[j].kind == EDPP_JOB_SPLIT) add(j, kPhaseScatter, dj[j].tiles);
+        }
+        if (tk >= (1ll << 32) - (1ll << 24))
+            return fail(c, EDPP_EUNSUP, "batch queue of %lld tiles is too large", (long long)tk);
+        qt.total[q] = tk;
+        qt.seg_count[q] = (int32_t)segs.size() - qt.seg_first[q];
+        total += tk;
     }
-    if (tk >= (1ll << 32)) return fail(c, EDPP_EUNSUP, "batch of %lld tiles is too large", (long long)tk);
-    b->total = tk;
+    b->qt = qt;
+    b->total = total;
     b->hist_tiles = hist;
     b->hscan_tiles = hscan;
-    b->nseg = (int32_t)segs.size();
     b->state_words = words;
     if (hipMalloc(&b->state, (size_t)words * 4) != hipSuccess) return fail(c, EDPP_ENOMEM, "batch state");
     for (int32_t j = 0; j < njobs; ++j) {
@@ -578,12 +636,11 @@ int edpp_batch_set_shape(edpp_batch* b, int32_t shape) {
 
 int edpp_batch_set_order(edpp_batch* b, int32_t order) {
     if (!b) return EDPP_EINVAL;
-    if (order != EDPP_BATCH_ORDER_PHASES && order != EDPP_BATCH_ORDER_JOBS)
+    if (order < EDPP_BATCH_ORDER_PHASES || order > EDPP_BATCH_ORDER_XCD)
         return fail(b->ctx, EDPP_EINVAL, "batch order %d unknown", order);
-    const bool il = order == EDPP_BATCH_ORDER_JOBS;
-    if (il == b->interleave && b->state) return EDPP_OK;
+    if (order == b->order && b->state) return EDPP_OK;
     EDPP_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));  // a run may still use the old plan
-    b->interleave = il;
+    b->order = order;
     return plan(b);
 }
 
@@ -591,7 +648,7 @@ int edpp_batch_get_shape(edpp_batch* b, int32_t* shape, int32_t* wgs_per_cu, int
     if (!b) return EDPP_EINVAL;
     if (shape) *shape = b->shape;
     if (wgs_per_cu) *wgs_per_cu = b->wgs_per_cu;
-    if (order) *order = b->interleave ? EDPP_BATCH_ORDER_JOBS : EDPP_BATCH_ORDER_PHASES;
+    if (order) *order = b->order;
     return EDPP_OK;
 }
 
@@ -633,7 +690,7 @@ int edpp_batch_run_graph(edpp_batch* b, double* ms) {
     return timing_end(c, ms);
 }
 
-// Exhaustive autotune of the batch's launch: both queue orders, every tile
+// Exhaustive autotune of the batch's launch: every queue order, every tile
 // shape, every occupancy of kOccupancies, one warm-up run and `reps` timed
 // runs each, the median kept; the fastest is left set.  Each run recomputes
 // every job's output from its inputs, so no job may write its own input.
@@ -646,9 +703,9 @@ int edpp_batch_autotune(edpp_batch* b, int32_t reps, double* times) {
             return fail(c, EDPP_EINVAL, "batch autotune: job %zu works in place, repeated runs would change it", j);
     }
     double best = -1.0;
-    int best_shape = b->shape, best_occ = b->wgs_per_cu, best_order = b->interleave ? 1 : 0;
+    int best_shape = b->shape, best_occ = b->wgs_per_cu, best_order = b->order;
     std::vector<double> ts((size_t)reps);
-    for (int ord = 0; ord < 2; ++ord) {
+    for (int ord = 0; ord < kNumOrders; ++ord) {
         int rc = edpp_batch_set_order(b, ord);
         if (rc) return rc;
         for (int s = 0; s < kNumShapes; ++s) {

@@ -64,19 +64,22 @@ int edpp_batch_shape_info(int32_t shape, int32_t* block, int32_t* scan_tile32, i
                           int32_t* split_tile, int32_t* gather_tile);
 int edpp_batch_set_shape(edpp_batch* b, int32_t shape);
 /* Queue order of the launch: every split's scatter tiles after all scan and
- * gather tiles (EDPP_BATCH_ORDER_PHASES, the default), or each job's main
- * tiles in job order, split scatters included (EDPP_BATCH_ORDER_JOBS), so
- * that streaming tiles run beside random-gather tiles.  Re-plans the queue;
- * results do not depend on it. */
-enum edpp_batch_order { EDPP_BATCH_ORDER_PHASES = 0, EDPP_BATCH_ORDER_JOBS = 1 };
+ * gather tiles (EDPP_BATCH_ORDER_PHASES), or each job's main tiles in job
+ * order, split scatters included (EDPP_BATCH_ORDER_JOBS), so that streaming
+ * tiles run beside random-gather tiles, or the jobs dealt to one queue per
+ * XCD, each queue in job order (EDPP_BATCH_ORDER_XCD, the default), so that
+ * all tiles of a job run on one XCD and share its L2; a workgroup whose
+ * queue is used up takes tiles from the next.  Re-plans the queue; results
+ * do not depend on it. */
+enum edpp_batch_order { EDPP_BATCH_ORDER_PHASES = 0, EDPP_BATCH_ORDER_JOBS = 1, EDPP_BATCH_ORDER_XCD = 2 };
 int edpp_batch_set_order(edpp_batch* b, int32_t order);
 int edpp_batch_get_shape(edpp_batch* b, int32_t* shape, int32_t* wgs_per_cu, int32_t* order);
 /* Exhaustive autotune of one batch, as the reference's launch-parameter
  * search tries every local and grid size of a split and keeps the fastest
- * (opencl/test/test_split.cpp:273-358): both queue orders, every shape, at
- * 2, 4 and 8 workgroups per compute unit, one warm-up run and `reps` timed
- * runs each (median); the fastest setting stays.  times (may be NULL)
- * receives 2 * edpp_batch_num_shapes() * 3 medians in ms, indexed
+ * (opencl/test/test_split.cpp:273-358): the three queue orders, every
+ * shape, at 2, 4 and 8 workgroups per compute unit, one warm-up run and
+ * `reps` timed runs each (median); the fastest setting stays.  times (may be
+ * NULL) receives 3 * edpp_batch_num_shapes() * 3 medians in ms, indexed
  * [order][shape][occupancy].  Runs the batch repeatedly, so jobs writing
  * their own input are rejected (EDPP_EINVAL). */
 int edpp_batch_autotune(edpp_batch* b, int32_t reps, double* times);

@@ -90,7 +90,7 @@ def test_batch_every_shape_matches_oracle(torch_gpu, ctx, orc, edpp):
             batch.set_shape(k)
             for occ in (2, 8):
                 batch.set_occupancy(occ)
-                order = (k + occ) % 2  # both queue orders over the shapes
+                order = (k + occ) % 3  # every queue order over the shapes
                 batch.set_order(order)
                 assert batch.get_shape() == (k, occ, order)
                 for bb in bufs:
@@ -111,7 +111,7 @@ def test_batch_autotune_picks_a_timed_shape(torch_gpu, ctx, orc, edpp):
     batch = edpp.Batch(ctx, jobsets[0])
     try:
         times = batch.autotune(reps=2)
-        assert len(times) == 2 * len(edpp.batch_shapes()) * len(edpp.BATCH_OCCUPANCIES)
+        assert len(times) == len(edpp.BATCH_ORDERS) * len(edpp.batch_shapes()) * len(edpp.BATCH_OCCUPANCIES)
         assert all(t > 0 for t in times.values())
         chosen = batch.get_shape()
         assert times[chosen] == min(times.values())
