@@ -312,7 +312,8 @@ int edpp_set_tuning(edpp_ctx* c, const edpp_tuning* t) {
     v.split_tile_mult = pick(v.split_tile_mult, d.split_tile_mult);
     v.scan_block = pick(v.scan_block, d.scan_block);
     if ((v.gs_unroll != 1 && v.gs_unroll != 2 && v.gs_unroll != 4 && v.gs_unroll != 8) || v.gs_random_nt > 1 ||
-        (v.scan_vpt != 2 && v.scan_vpt != 4 && v.scan_vpt != 8 && v.scan_vpt != 16) || (v.split_ipt != 8 && v.split_ipt != 16) ||
+        (v.scan_vpt != 0 && v.scan_vpt != 2 && v.scan_vpt != 4 && v.scan_vpt != 8 && v.scan_vpt != 16) ||
+        (v.split_ipt != 8 && v.split_ipt != 16) ||
         v.split_tile_mult < 16 || v.split_tile_mult > 1024 ||
         (v.scan_block != 256 && v.scan_block != 512 && v.scan_block != 1024) ||
         (v.split_block != 0 && v.split_block != 256 && v.split_block != 512) ||

@@ -49,7 +49,7 @@ struct edpp_ctx {
         edpp_tuning t = {};
         t.gs_unroll = 4;
         t.gs_random_nt = 0;
-        t.scan_vpt = 8;
+        t.scan_vpt = 0;  // by size (default_scan_shape)
         t.scan_block = 1024;
         t.split_ipt = 16;
         t.split_tile_mult = 16;

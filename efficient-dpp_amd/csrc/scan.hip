@@ -304,7 +304,12 @@ int launch_scan(edpp_ctx* c, const T* in, T* out, int64_t n, int scheme, T* d_to
     // the three-kernel schemes, whose partial scan is the serial bottleneck
     switch (scheme) {
         case EDPP_SCAN_LOOKBACK: {
-            const int v = c->tune.scan_vpt, b = c->tune.scan_block;
+            int v = c->tune.scan_vpt;
+            const int b = c->tune.scan_block;
+            if (v == 0) {  // vectors per lane by size
+                int unused_block;
+                default_scan_shape(c, n, &unused_block, &v);
+            }
             if (b == 512) {
                 if (v <= 4) return run_lookback<T, 512, 4>(c, in, out, n, d_total);
                 if (v == 8) return run_lookback<T, 512, 8>(c, in, out, n, d_total);

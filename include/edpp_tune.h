@@ -21,7 +21,8 @@ extern "C" {
 typedef struct edpp_tuning {
     int32_t gs_unroll;        /* index vectors (4 elements each) per lane: 1, 2, 4, 8; 0 = default      */
     int32_t gs_random_nt;     /* random side of gather/scatter with non-temporal hint: 0 / 1; -1 default */
-    int32_t scan_vpt;         /* 16-byte vectors per lane in a look-back tile: 2, 4, 8, 16; 0 = default  */
+    int32_t scan_vpt;         /* 16-byte vectors per lane in a look-back tile: 2, 4, 8, 16; 0 = by size
+                                 (4 below 2^21 elements, 8 below 2^23, 16 above; 64-bit scans stop at 8) */
     int32_t split_ipt;        /* keys per lane per split sub-tile: 8 or 16; 0 = default                  */
     int32_t split_tile_mult;  /* split tile >= mult * buckets elements: 16..1024; 0 = default (16)       */
     int32_t scan_block;       /* lanes per look-back workgroup: 256, 512, 1024; 0 = default              */
