@@ -155,15 +155,15 @@ __global__ void k_split_start(const int32_t* scanned, int64_t tiles, int B, int3
 // A pass counts its digit per range of tiles (bucket totals of each range,
 // not of each tile); then its scatter finds each tile's bucket offsets by a
 // look-back over predecessor tiles' per-bucket counts instead of a
-// [B][tiles] matrix written by a histogram kernel and scanned.  The tiles are
-// cut into one
-// contiguous range per XCD: workgroup g takes tiles in order from range
-// g % xcds (the XCD it runs on), so neighbouring tiles, whose runs of a bucket
-// share cache lines, are written through one L2, and a look-back never leaves
-// its range, whose base offset per bucket the counts give.  A workgroup whose
-// range is used up takes a tile from another range; since every range hands
-// out its tiles in order and a tile waits only for earlier tiles of its own
-// range, each of which belongs to a running workgroup, every wait ends.
+// [B][tiles] matrix written by a histogram kernel and scanned.  The tiles
+// are cut into one contiguous range per XCD: workgroup g takes tiles in
+// order from range g % xcds (the XCD it runs on), so neighbouring tiles,
+// whose runs of a bucket share cache lines, are written through one L2, and
+// a look-back never leaves its range, whose base offset per bucket the
+// counts give.  A workgroup whose range is used up takes a tile from another
+// range; since every range hands out its tiles in order and a tile waits
+// only for earlier tiles of its own range, each of which belongs to a
+// running workgroup, every wait ends.
 
 constexpr int kChainMaxDigits = 4;  // 32 key bits in digits of at most 8
 constexpr int kChainMaxRanges = 8;
@@ -188,9 +188,10 @@ struct ChainDigits {
 };
 
 // counts[(d*ranges + r)*kChainStride + b] += number of keys of range r whose
-// digit d is b (the kernel counts up to kChainMaxDigits digits per read).  Workgroup j of range r counts the j-th of wpr contiguous
-// parts of the range, keys loaded once for every digit, per-wave LDS rows
-// [waves][digits][kChainStride]; one atomic per nonzero count at the end.
+// digit d is b, for up to kChainMaxDigits digits per read of the keys.
+// Workgroup j of range r counts the j-th of wpr contiguous parts of the
+// range into per-wave LDS rows [waves][digits][kChainStride]; one atomic per
+// nonzero count at the end.
 template <class K, int IPT>
 __global__ __launch_bounds__(kSplitBlock) void k_chain_hist(const K* __restrict__ kin, int64_t n, int64_t tile,
                                                             ChainRanges cr, int wpr, ChainDigits dg,
