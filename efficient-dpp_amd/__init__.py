@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libedpp.so")
 # ---- status codes and enums (mirror include/edpp.h) -----------------------
 OK, EINVAL, ENOMEM, ETIMEOUT, EHIP, ENODEV, EUNSUP = 0, -1, -2, -3, -4, -5, -6
 
-SCAN_AUTO, SCAN_LOOKBACK, SCAN_RTS, SCAN_SSA = 0, 1, 2, 3
+SCAN_AUTO, SCAN_LOOKBACK, SCAN_RTS, SCAN_SSA, SCAN_PERSIST = 0, 1, 2, 3, 4
 MATRIX_LM, MATRIX_REG, MATRIX_LM_REG, MATRIX_LM_SERIAL = 0, 1, 2, 3
 NET_SERIAL, NET_BLELLOCH, NET_SKLANSKY, NET_WAVE = 0, 1, 2, 3
 NET_SERIAL_MT, NET_BLELLOCH_CF, NET_GLOBAL_BLELLOCH, NET_GLOBAL_SKLANSKY, NET_GLOBAL_WAVE = 4, 5, 6, 7, 8

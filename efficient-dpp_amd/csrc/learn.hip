@@ -62,7 +62,7 @@ const Cand kCands[K] = {
     {0, "scan look-back 1024x4", EDPP_SCAN_LOOKBACK, 1024, 4},
     {0, "scan look-back 1024x16", EDPP_SCAN_LOOKBACK, 1024, 16},
     {0, "scan reduce-then-scan", EDPP_SCAN_RTS, 0, 0},
-    {0, "scan scan-scan-add", EDPP_SCAN_SSA, 0, 0},
+    {0, "scan persistent look-back 1024x8", EDPP_SCAN_PERSIST, 1024, 8},
     {1, "gather/scatter 1 pass, unroll 4", 1, 4, 0},
     {1, "gather/scatter 1 pass, unroll 8", 1, 8, 0},
     {1, "gather/scatter 2 passes", 2, 4, 0},
@@ -489,7 +489,7 @@ int run_candidate(edpp_ctx* c, const edpp_call& q, int cand, void* aux) {
     int rc;
     const bool w = q.elem_bytes == 8;
     if (k.family == 0) {
-        if (k.scheme == EDPP_SCAN_LOOKBACK) {
+        if (k.scheme == EDPP_SCAN_LOOKBACK || k.scheme == EDPP_SCAN_PERSIST) {
             c->tune.scan_block = k.knob_a;
             c->tune.scan_vpt = k.knob_b;
         }
@@ -657,6 +657,7 @@ int edpp_default_candidate(edpp_ctx* c, const edpp_call* q, int32_t* cand) {
     if (fam == 0) {
         scheme = default_scan_scheme(c, q->n, q->elem_bytes);
         if (scheme == EDPP_SCAN_LOOKBACK) default_scan_shape(c, q->n, &a, &b);
+        if (scheme == EDPP_SCAN_PERSIST) a = 1024, b = 8;
     } else if (fam == 1) {
         scheme = default_passes(c, q->n, q->elem_bytes, q->op == EDPP_OP_SCATTER);
         a = c->tune.gs_unroll;
@@ -669,7 +670,7 @@ int edpp_default_candidate(edpp_ctx* c, const edpp_call* q, int32_t* cand) {
         const Cand& d = kCands[k];
         if (d.family != fam || d.scheme != scheme) continue;
         // knobs a scheme does not read match anything
-        const bool lb = fam == 0 && scheme == EDPP_SCAN_LOOKBACK;
+        const bool lb = fam == 0 && (scheme == EDPP_SCAN_LOOKBACK || scheme == EDPP_SCAN_PERSIST);
         const bool ka = fam == 0 ? (!lb || d.knob_a == a) : d.knob_a == a;
         const bool kb = fam == 1 ? true : (fam == 0 ? (!lb || d.knob_b == b) : d.knob_b == b);
         if (ka && kb) {

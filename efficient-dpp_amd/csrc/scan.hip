@@ -60,6 +60,71 @@ __global__ __launch_bounds__(BLOCK) void k_scan_lookback(const T* in, T* out,
     ts.store_folded(out, base, n, sm.prefix);
 }
 
+// Persistent form of the look-back scan, for calls with several tiles per
+// workgroup: each workgroup loops over tickets and loads its next tile while
+// the current one looks back and stores, so the look-back's round trips and
+// the ticket's atomic overlap memory traffic.  Per step: take the next
+// ticket, issue its loads, look back for the current tile and publish its
+// prefix, scan the next tile and publish its aggregate, store the current
+// tile.  A workgroup holds two tickets at once, but the smaller one has
+// published its prefix before the larger one is waited on, and the larger
+// one's aggregate needs nothing that is not finished, so every wait ends.
+template <class T, int BLOCK, int VPT>
+__global__ __launch_bounds__(BLOCK) void k_scan_pipe(const T* in, T* out, int64_t n,
+                                                     typename LookbackFor<T>::type st, uint32_t* tickets,
+                                                     T* d_total, uint32_t spin_limit, int32_t* err, int64_t tiles) {
+    typedef TileScan<T, BLOCK, VPT> TS;
+    __shared__ typename TS::Smem sm;
+    __shared__ uint32_t s_ticket;
+    if (threadIdx.x == 0) {
+        const uint32_t t = fetch_add(tickets + (st.epoch & 1u), 1u);
+        // the next call uses the other counter; it starts after this grid ends
+        if (t == 0) st_relaxed(tickets + ((st.epoch + 1u) & 1u), 0u);
+        s_ticket = t;
+    }
+    __syncthreads();
+    int64_t cur = s_ticket;
+    if (cur >= tiles) return;  // more workgroups than tiles
+    TS a;
+    a.load(in, cur * TS::TILE, n);
+    a.scan(sm);
+    a.fold();
+    if (threadIdx.x == 0) st.publish(cur, cur == 0 ? kFlagPrefix : kFlagAggregate, a.total);
+    while (true) {
+        if (threadIdx.x == 0) {
+            const uint32_t t = fetch_add(tickets + (st.epoch & 1u), 1u);
+            if (t == 0) st_relaxed(tickets + ((st.epoch + 1u) & 1u), 0u);
+            s_ticket = t;
+        }
+        __syncthreads();
+        const int64_t nxt = s_ticket;
+        TS b;
+        if (nxt < tiles) b.load(in, nxt * TS::TILE, n);  // in flight through the look-back below
+        if (threadIdx.x < kWave) {
+            T prefix = 0;
+            if (cur > 0) {
+                prefix = lookback<T>(st, cur, spin_limit, err);
+                if (threadIdx.x == 0) st.publish(cur, kFlagPrefix, prefix + a.total);
+            }
+            if (threadIdx.x == 0) {
+                sm.prefix = prefix;
+                if (d_total && cur * TS::TILE + TS::TILE >= n) *d_total = prefix + a.total;
+            }
+        }
+        __syncthreads();
+        const T prefix = sm.prefix;
+        if (nxt < tiles) {
+            b.scan(sm);  // leaves sm.prefix alone; its barriers order every read of it before the next write
+            b.fold();
+            if (threadIdx.x == 0) st.publish(nxt, nxt == 0 ? kFlagPrefix : kFlagAggregate, b.total);
+        }
+        a.store_folded(out, cur * TS::TILE, n, prefix);
+        if (nxt >= tiles) break;
+        a = b;
+        cur = nxt;
+    }
+}
+
 // ---------------------------------------------------------------- RTS / SSA pieces
 // partials[tile] = sum of the tile
 template <class T, int BLOCK, int VPT>
@@ -242,6 +307,24 @@ int run_lookback(edpp_ctx* c, const T* in, T* out, int64_t n, T* d_total) {
     return EDPP_OK;
 }
 
+// Persistent look-back scan: 1024-lane workgroups holding two tiles of
+// registers each, as many as fit on the chip at once (two per CU for 4
+// vectors of 32-bit values per lane, else one).
+template <class T, int VPT>
+int run_pipe(edpp_ctx* c, const T* in, T* out, int64_t n, T* d_total) {
+    typedef TileScan<T, 1024, VPT> TS;
+    const int64_t tiles = ceil_div(n, TS::TILE);
+    uint32_t epoch;
+    int rc = scan_state_acquire(c, tiles, &epoch);
+    if (rc) return rc;
+    const int per_cu = (sizeof(T) == 4 && VPT <= 4) ? 2 : 1;
+    const int64_t grid = std::min<int64_t>(tiles, (int64_t)std::max(c->cus, 1) * per_cu);
+    hipLaunchKernelGGL((k_scan_pipe<T, 1024, VPT>), dim3((unsigned)grid), dim3(1024), 0, c->stream, in, out, n,
+                       make_state<T>(c, epoch), c->tickets, d_total, c->spin_limit, c->d_err, tiles);
+    EDPP_HIP(c, hipGetLastError());
+    return EDPP_OK;
+}
+
 template <class T>
 int run_unaligned(edpp_ctx* c, const T* in, T* out, int64_t n, T* d_total) {
     int64_t tiles = ceil_div(n, 64 * 16);
@@ -300,8 +383,8 @@ int launch_scan(edpp_ctx* c, const T* in, T* out, int64_t n, int scheme, T* d_to
             return rc;
         }
     }
-    // tile of 4096 (u32) / 2048 (u64) elements for look-back; 8192 / 4096 for
-    // the three-kernel schemes, whose partial scan is the serial bottleneck
+    // look-back tiles of lanes x 4..16 vectors per lane (c->tune); 2048-lane
+    // tiles of 8 vectors for the three-kernel schemes
     switch (scheme) {
         case EDPP_SCAN_LOOKBACK: {
             int v = c->tune.scan_vpt;
@@ -310,6 +393,7 @@ int launch_scan(edpp_ctx* c, const T* in, T* out, int64_t n, int scheme, T* d_to
                 int unused_block;
                 default_scan_shape(c, n, &unused_block, &v);
             }
+
             if (b == 512) {
                 if (v <= 4) return run_lookback<T, 512, 4>(c, in, out, n, d_total);
                 if (v == 8) return run_lookback<T, 512, 8>(c, in, out, n, d_total);
@@ -327,6 +411,9 @@ int launch_scan(edpp_ctx* c, const T* in, T* out, int64_t n, int scheme, T* d_to
             if (v == 8) return run_lookback<T, 256, 8>(c, in, out, n, d_total);
             return run_lookback<T, 256, 16>(c, in, out, n, d_total);
         }
+        case EDPP_SCAN_PERSIST:  // 1024 lanes, 4 or 8 vectors per lane (8 unless 4 is asked)
+            if (c->tune.scan_vpt == 4) return run_pipe<T, 4>(c, in, out, n, d_total);
+            return run_pipe<T, 8>(c, in, out, n, d_total);
         case EDPP_SCAN_RTS: return run_rts_ssa<T, 8>(c, in, out, n, d_total, false);
         case EDPP_SCAN_SSA: return run_rts_ssa<T, 8>(c, in, out, n, d_total, true);
         default: return fail(c, EDPP_EINVAL, "unknown scan scheme %d", scheme);

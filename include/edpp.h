@@ -60,7 +60,9 @@ enum edpp_scan_scheme {          /* SCAN_GLOBAL_TYPE, opencl/types.h:8-10 */
     EDPP_SCAN_AUTO     = 0,
     EDPP_SCAN_LOOKBACK = 1,      /* single pass, decoupled look-back (replaces CHAINED)            */
     EDPP_SCAN_RTS      = 2,      /* reduce -> scan partials -> scan tiles with offsets (RSS)       */
-    EDPP_SCAN_SSA      = 3       /* scan tiles -> scan partials -> add offsets                     */
+    EDPP_SCAN_SSA      = 3,      /* scan tiles -> scan partials -> add offsets                     */
+    EDPP_SCAN_PERSIST  = 4       /* the look-back on persistent workgroups, each loading its next
+                                    tile while the current one looks back (large calls)             */
 };
 
 enum edpp_matrix_scan {          /* MatrixScanType, opencl/types.h:12-14 */

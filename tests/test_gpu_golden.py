@@ -32,7 +32,7 @@ def test_golden_scan(torch_gpu, ctx):
     for k in cases(z, "_in"):
         x, y = z[k + "_in"], z[k + "_out"]
         wide = x.dtype == np.int64
-        for scheme in (1, 2, 3):
+        for scheme in (1, 2, 3, 4):
             out = torch.empty(len(x), dtype=torch.int64 if wide else torch.int32, device="cuda")
             tot = torch.zeros(1, dtype=out.dtype, device="cuda")
             ctx.scan(dev(torch, x), out, len(x), scheme=scheme, total=tot, wide=wide)

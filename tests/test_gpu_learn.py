@@ -178,6 +178,7 @@ def test_default_candidate_is_the_builtin_schedule(torch_gpu, ctx, edpp):
         (0, 1 << 22, 8, 256): "scan look-back 1024x8",
         (0, 1 << 24, 4, 256): "scan look-back 1024x16",
         (0, 1 << 24, 8, 256): "scan look-back 1024x16",  # runs 1024x8 for 64-bit values
+        (0, 1 << 25, 4, 256): "scan persistent look-back 1024x8",
         (1, 1 << 20, 4, 256): "gather/scatter 1 pass, unroll 4",
         (2, 1 << 25, 4, 256): "gather/scatter 2 passes",
         (3, 1 << 20, 4, 256): "split WG 8/lane, tile 16B",

@@ -65,7 +65,7 @@ def test_stream_ops(torch_gpu, ctx, orc, n):
 
 
 # ---------------------------------------------------------------- scans
-@pytest.mark.parametrize("scheme", [1, 2, 3])
+@pytest.mark.parametrize("scheme", [1, 2, 3, 4])
 @pytest.mark.parametrize("n", SIZES)
 def test_scan_i32(torch_gpu, ctx, orc, scheme, n):
     torch = torch_gpu
@@ -80,7 +80,7 @@ def test_scan_i32(torch_gpu, ctx, orc, scheme, n):
         assert int(host(total)[0]) == int(tot)
 
 
-@pytest.mark.parametrize("scheme", [1, 2, 3])
+@pytest.mark.parametrize("scheme", [1, 2, 3, 4])
 @pytest.mark.parametrize("n", [1, 4097, 1 << 20, 3_000_001])
 def test_scan_i64(torch_gpu, ctx, orc, scheme, n):
     torch = torch_gpu
@@ -93,11 +93,13 @@ def test_scan_i64(torch_gpu, ctx, orc, scheme, n):
     assert int(host(total)[0]) == int(tot)
 
 
-@pytest.mark.parametrize("block,vpt", [(256, 2), (256, 4), (256, 8), (256, 16), (512, 4), (512, 8), (512, 16),
-                                       (1024, 4), (1024, 8), (1024, 16)])
-def test_scan_lookback_shapes(torch_gpu, ctx, orc, block, vpt):
-    # every look-back tile shape (lanes x 16-byte vectors per lane), 32- and
-    # 64-bit, whole and ragged last tiles, in place
+@pytest.mark.parametrize("block,vpt,scheme", [(256, 2, 1), (256, 4, 1), (256, 8, 1), (256, 16, 1), (512, 4, 1),
+                                              (512, 8, 1), (512, 16, 1), (1024, 4, 1), (1024, 8, 1), (1024, 16, 1),
+                                              (1024, 4, 4), (1024, 8, 4)])
+def test_scan_lookback_shapes(torch_gpu, ctx, orc, block, vpt, scheme):
+    # every look-back tile shape (lanes x 16-byte vectors per lane), one
+    # workgroup per tile (scheme 1) or persistent workgroups (scheme 4),
+    # 32- and 64-bit, whole and ragged last tiles, in place
     torch = torch_gpu
     ctx.set_tuning(scan_block=block, scan_vpt=vpt)
     try:
@@ -106,23 +108,23 @@ def test_scan_lookback_shapes(torch_gpu, ctx, orc, block, vpt):
             ref, tot = orc.scan_exclusive(x)
             out = empty(torch, n, torch.int32)
             total = torch.zeros(1, dtype=torch.int32, device="cuda")
-            ctx.scan(dev(torch, x), out, n, scheme=1, total=total)
+            ctx.scan(dev(torch, x), out, n, scheme=scheme, total=total)
             assert np.array_equal(host(out), ref), (block, vpt, n)
             assert int(host(total)[0]) == int(tot)
             d = dev(torch, x)
-            ctx.scan(d, d, n, scheme=1)
+            ctx.scan(d, d, n, scheme=scheme)
             assert np.array_equal(host(d), ref), ("in place", block, vpt, n)
         for n in (4097, 3_000_001):
             x = orc.gen_i64(n, 17 + n + vpt)
             ref, tot = orc.scan_exclusive(x)
             out = empty(torch, n, torch.int64)
-            ctx.scan(dev(torch, x), out, n, scheme=1, wide=True)
+            ctx.scan(dev(torch, x), out, n, scheme=scheme, wide=True)
             assert np.array_equal(host(out), ref), ("i64", block, vpt, n)
     finally:
         ctx.set_tuning()
 
 
-@pytest.mark.parametrize("scheme", [1, 2, 3])
+@pytest.mark.parametrize("scheme", [1, 2, 3, 4])
 def test_scan_in_place_and_unaligned(torch_gpu, ctx, orc, scheme):
     torch = torch_gpu
     n = 1_000_003
@@ -138,7 +140,7 @@ def test_scan_in_place_and_unaligned(torch_gpu, ctx, orc, scheme):
 
 
 def test_scan_c1_config(torch_gpu, ctx, orc):
-    # 2^24 int32 values uniform in [0,15] from a fixed seed, all three schemes
+    # 2^24 int32 values uniform in [0,15] from a fixed seed, every scheme
     torch = torch_gpu
     n = 1 << 24
     x = orc.gen_i32(n, 2024, 15)
@@ -147,7 +149,7 @@ def test_scan_c1_config(torch_gpu, ctx, orc):
     ctx.gen_i32(d, 2024, 15)
     out = empty(torch, n, torch.int32)
     total = torch.zeros(1, dtype=torch.int32, device="cuda")
-    for scheme in (1, 2, 3):
+    for scheme in (1, 2, 3, 4):
         out.fill_(-1)
         ctx.scan(d, out, scheme=scheme, total=total)
         assert np.array_equal(host(out), ref), scheme
@@ -163,7 +165,7 @@ def test_scan_full_size_properties(torch_gpu, ctx):
     ctx.gen_i32(x, 77, 15)
     out = empty(torch, n, torch.int32)
     total = torch.zeros(1, dtype=torch.int32, device="cuda")
-    for scheme in (1, 2, 3):
+    for scheme in (1, 2, 3, 4):
         ctx.scan(x, out, scheme=scheme, total=total)
         assert int(out[0]) == 0
         assert torch.equal(out[1:] - out[:-1], x[:-1])  # int32 wrap-around subtraction
