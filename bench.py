@@ -276,7 +276,7 @@ def profile_only(args, ctx, edpp, torch):
     elif args.profile == "scan":
         ctx.gen_i32(a, 2024, 15)
         for _ in range(reps):
-            ctx.scan(a, b, n, edpp.SCAN_LOOKBACK)
+            ctx.scan(a, b, n, edpp.SCAN_AUTO)
     elif args.profile == "split":
         v = torch.empty(n, dtype=i32, device="cuda")
         vo = torch.empty(n, dtype=i32, device="cuda")
@@ -609,7 +609,7 @@ def suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs):
         m = 1 << lg
         x = b[:m]
         ctx.gen_i32(x, 2024, 15)
-        for name, scheme in [("lookback", 1), ("rts", 2), ("ssa", 3)]:
+        for name, scheme in [("lookback", 1), ("rts", 2), ("ssa", 3), ("persist", 4)]:
             t = T.time(lambda: ctx.scan(x, dst, m, scheme), reps=20)
             tc = T.time(lambda: ctx.scan(x, dst, m, scheme), reps=20, flush=flush)
             g = gbs(8.0 * m, t["hampel_ms"])
@@ -715,6 +715,8 @@ def suite(ctx, edpp, torch, T, src, dst, idx, copy_gbs):
     wide = {}
     t = T.time(lambda: ctx.scan(wa, wb, m, edpp.SCAN_LOOKBACK, wide=True), reps=20)
     wide["scan_lookback"] = {"gbs": round(gbs(16.0 * m, t["hampel_ms"]), 2), "ms": round(t["hampel_ms"], 4)}
+    t = T.time(lambda: ctx.scan(wa, wb, m, edpp.SCAN_AUTO, wide=True), reps=20)
+    wide["scan_auto"] = {"gbs": round(gbs(16.0 * m, t["hampel_ms"]), 2), "ms": round(t["hampel_ms"], 4)}
     widx = idx[:m]
     ctx.gen_perm(widx, 0x5EED, edpp.PERM_RANDOM, n=m)
     for p in (1, 4):
