@@ -43,20 +43,22 @@ struct DJob {
     uint64_t* his;     // split: [B][tiles] tagged words {tag:32 | value:32}
 };
 
-struct Seg {
-    int64_t first;  // first ticket within its queue
-    int32_t job, phase;
-};
-
 constexpr int kMaxQueues = 8;
 constexpr int kTicketStride = 32;  // 32-bit words between the queues' ticket counters (one 128-byte line each)
 
-// The queues of a launch: queue q holds segments [seg_first[q], seg_first[q]
-// + seg_count[q]) of the segment table and total[q] tickets.
+// One 64-bit word per ticket names its tile: job (bits 34..63), phase (32..33)
+// and the tile's index within the job and phase (0..31), so a workgroup finds
+// its work with one load.
+__host__ __device__ constexpr uint64_t pack_ticket(uint32_t job, uint32_t phase, uint32_t tile) {
+    return ((uint64_t)job << 34) | ((uint64_t)phase << 32) | tile;
+}
+constexpr int64_t kMaxBatchTiles = 1ll << 27;  // a 1 GiB ticket table
+
+// The queues of a launch: queue q's tickets are entries [tix_first[q],
+// tix_first[q] + total[q]) of the ticket table.
 struct QTable {
     int64_t total[kMaxQueues];
-    int32_t seg_first[kMaxQueues];
-    int32_t seg_count[kMaxQueues];
+    int64_t tix_first[kMaxQueues];
     int32_t nq, xcds;
 };
 
@@ -302,11 +304,10 @@ __global__ void k_batch_init(uint32_t* state, int64_t words, const DJob* jobs, i
 }
 
 template <int BLOCK, int SV32, int SV64, int IPT, int GU>
-__global__ __launch_bounds__(BLOCK) void k_batch(const DJob* __restrict__ jobs, const Seg* __restrict__ segs,
+__global__ __launch_bounds__(BLOCK) void k_batch(const DJob* __restrict__ jobs, const uint64_t* __restrict__ tix,
                                                  QTable qt, uint32_t* tickets, uint32_t spin_limit, int32_t* err) {
     extern __shared__ __align__(16) char smem[];
     __shared__ uint32_t s_ticket;
-    __shared__ int32_t s_seg;
     int q = qt.nq == 1 ? 0 : (int)(blockIdx.x % (unsigned)qt.xcds) % qt.nq;  // this XCD's queue
     int spent = 0;  // queues found used up
     while (true) {
@@ -319,33 +320,17 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const DJob* __restrict__ jobs, 
             q = q + 1 == qt.nq ? 0 : q + 1;
             continue;
         }
-        // wave 0 finds the last segment of the queue starting at or before
-        // tk: 64 candidates per round trip, so about three rounds for the
-        // 10 000-job mix's 20 000-odd segments instead of 15 dependent loads
-        if (threadIdx.x < kWave) {
-            const int lane = lane_id();
-            int lo = qt.seg_first[q], hi = qt.seg_first[q] + qt.seg_count[q] - 1;
-            while (lo < hi) {
-                const int step = (hi - lo + kWave) / kWave;  // ceil((hi - lo + 1) / 64)
-                const int at = lo + lane * step;
-                const uint64_t m = ballot(at <= hi && segs[at].first <= tk);  // lane 0 always: segs[lo].first <= tk
-                const int l = 63 - (int)__builtin_clzll(m);
-                lo += l * step;
-                hi = std::min(hi, lo + step - 1);
-            }
-            if (lane == 0) s_seg = lo;
-        }
-        __syncthreads();
-        const Seg sg = segs[s_seg];
-        const DJob& J = jobs[sg.job];
-        const int64_t t = tk - sg.first;
+        const uint64_t e = tix[qt.tix_first[q] + tk];
+        const int32_t phase = (int32_t)((e >> 32) & 3u);
+        const DJob& J = jobs[e >> 34];
+        const int64_t t = (int64_t)(uint32_t)e;
         uint32_t* lds = reinterpret_cast<uint32_t*>(smem);
-        if (sg.phase == kPhaseHist) {
+        if (phase == kPhaseHist) {
             if (J.wide) split_hist_tile<uint64_t, IPT, BLOCK>(J, t, lds);
             else split_hist_tile<uint32_t, IPT, BLOCK>(J, t, lds);
-        } else if (sg.phase == kPhaseHistScan) {
+        } else if (phase == kPhaseHistScan) {
             hist_scan_tile<BLOCK>(J, t, smem, spin_limit, err);
-        } else if (sg.phase == kPhaseScatter) {
+        } else if (phase == kPhaseScatter) {
             if (J.wide) split_scatter_tile<uint64_t, IPT, BLOCK>(J, t, lds, spin_limit, err);
             else split_scatter_tile<uint32_t, IPT, BLOCK>(J, t, lds, spin_limit, err);
         } else if (J.kind == EDPP_JOB_SCAN) {
@@ -379,7 +364,7 @@ constexpr size_t shape_lds(const BatchShape& s) {
                             1024);
 }
 
-typedef void (*BatchKernel)(const DJob*, const Seg*, QTable, uint32_t*, uint32_t, int32_t*);
+typedef void (*BatchKernel)(const DJob*, const uint64_t*, QTable, uint32_t*, uint32_t, int32_t*);
 
 template <int I>
 constexpr BatchKernel kernel_of() {
@@ -403,7 +388,7 @@ struct edpp_batch {
     int order = EDPP_BATCH_ORDER_XCD;  // EDPP_BATCH_ORDER_*
     // the plan of the current shape
     DJob* d_jobs = nullptr;
-    Seg* d_segs = nullptr;
+    uint64_t* d_tix = nullptr;  // one word per ticket: job, phase, tile
     QTable qt = {};
     int64_t total = 0, hist_tiles = 0, hscan_tiles = 0;
     uint32_t* state = nullptr;  // zeroed each run: ticket, look-back status, split histogram words
@@ -424,10 +409,10 @@ static void drop_plan(edpp_batch* b) {
     drop_graph(b);
     hipFree(b->state);
     hipFree(b->d_jobs);
-    hipFree(b->d_segs);
+    hipFree(b->d_tix);
     b->state = nullptr;
     b->d_jobs = nullptr;
-    b->d_segs = nullptr;
+    b->d_tix = nullptr;
 }
 
 static int batch_launch(edpp_batch* b, hipStream_t s) {
@@ -439,7 +424,8 @@ static int batch_launch(edpp_batch* b, hipStream_t s) {
         const BatchShape& sh = kShapes[b->shape];
         const int64_t grid = std::min<int64_t>(b->total, (int64_t)c->cus * b->wgs_per_cu);
         hipLaunchKernelGGL(kKernels[b->shape], dim3((unsigned)grid), dim3(sh.block), shape_lds(sh), s,
-                           (const DJob*)b->d_jobs, (const Seg*)b->d_segs, b->qt, b->state, c->spin_limit, c->d_err);
+                           (const DJob*)b->d_jobs, (const uint64_t*)b->d_tix, b->qt, b->state, c->spin_limit,
+                           c->d_err);
     }
     EDPP_HIP(c, hipGetLastError());
     return EDPP_OK;
@@ -526,24 +512,26 @@ static int plan(edpp_batch* b) {
             load[qq] += cost[j];
         }
     }
-    // segments in queue order
-    std::vector<Seg> segs;
+    // the ticket table, queue after queue
+    int64_t all_tiles = 0;
+    for (int32_t j = 0; j < njobs; ++j)  // a split: histogram, histogram-scan and scatter tiles
+        all_tiles += dj[j].tiles + (dj[j].kind == EDPP_JOB_SPLIT ? dj[j].tiles + dj[j].hscan_tiles : 0);
+    if (all_tiles > kMaxBatchTiles)
+        return fail(c, EDPP_EUNSUP, "batch of %lld tiles is too large (at most %lld)", (long long)all_tiles,
+                    (long long)kMaxBatchTiles);
+    std::vector<uint64_t> tix;
+    tix.reserve((size_t)all_tiles);
     QTable qt = {};
     qt.nq = nq;
     qt.xcds = std::max(1, c->xcds);
     int64_t tk = 0, total = 0, hist = 0, hscan = 0;
     auto add = [&](int32_t j, int32_t phase, int64_t count) {
-        if (count <= 0) return;
-        Seg s;
-        s.first = tk;
-        s.job = j;
-        s.phase = phase;
-        segs.push_back(s);
-        tk += count;
+        for (int64_t i = 0; i < count; ++i) tix.push_back(pack_ticket((uint32_t)j, (uint32_t)phase, (uint32_t)i));
+        tk += std::max<int64_t>(count, 0);
     };
     for (int q = 0; q < nq; ++q) {
         tk = 0;
-        qt.seg_first[q] = (int32_t)segs.size();
+        qt.tix_first[q] = (int64_t)tix.size();
         for (int32_t j = 0; j < njobs; ++j)
             if (qof[j] == q && dj[j].kind == EDPP_JOB_SPLIT) { add(j, kPhaseHist, dj[j].tiles); hist += dj[j].tiles; }
         for (int32_t j = 0; j < njobs; ++j)
@@ -562,10 +550,7 @@ static int plan(edpp_batch* b) {
             for (int32_t j = 0; j < njobs; ++j)
                 if (qof[j] == q && dj[j].kind == EDPP_JOB_SPLIT) add(j, kPhaseScatter, dj[j].tiles);
         }
-        if (tk >= (1ll << 32) - (1ll << 24))
-            return fail(c, EDPP_EUNSUP, "batch queue of %lld tiles is too large", (long long)tk);
         qt.total[q] = tk;
-        qt.seg_count[q] = (int32_t)segs.size() - qt.seg_first[q];
         total += tk;
     }
     b->qt = qt;
@@ -585,11 +570,11 @@ static int plan(edpp_batch* b) {
         }
     }
     if (hipMalloc(&b->d_jobs, sizeof(DJob) * std::max<size_t>(1, dj.size())) != hipSuccess ||
-        hipMalloc(&b->d_segs, sizeof(Seg) * std::max<size_t>(1, segs.size())) != hipSuccess)
+        hipMalloc(&b->d_tix, sizeof(uint64_t) * std::max<size_t>(1, tix.size())) != hipSuccess)
         return fail(c, EDPP_ENOMEM, "batch job table");
     if ((njobs && hipMemcpy(b->d_jobs, dj.data(), sizeof(DJob) * dj.size(), hipMemcpyHostToDevice) != hipSuccess) ||
-        (!segs.empty() &&
-         hipMemcpy(b->d_segs, segs.data(), sizeof(Seg) * segs.size(), hipMemcpyHostToDevice) != hipSuccess))
+        (!tix.empty() &&
+         hipMemcpy(b->d_tix, tix.data(), sizeof(uint64_t) * tix.size(), hipMemcpyHostToDevice) != hipSuccess))
         return fail(c, EDPP_EHIP, "batch: upload of the job table failed");
     EDPP_HIP(c, hipFuncSetAttribute((const void*)kKernels[b->shape], hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)shape_lds(sh)));
