@@ -313,14 +313,18 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const DJob* __restrict__ jobs, 
     while (true) {
         if (threadIdx.x == 0) s_ticket = fetch_add(tickets + q * kTicketStride, 1u);
         __syncthreads();
-        const int64_t tk = s_ticket;
+        const int64_t tk = (uint32_t)__builtin_amdgcn_readfirstlane(s_ticket);
         if (tk >= qt.total[q]) {
             __syncthreads();  // every lane has read s_ticket before lane 0 writes it again
             if (++spent >= qt.nq) break;
             q = q + 1 == qt.nq ? 0 : q + 1;
             continue;
         }
-        const uint64_t e = tix[qt.tix_first[q] + tk];
+        // the same word in every lane: made uniform so that the job's fields
+        // come through scalar loads
+        const uint64_t ev = tix[qt.tix_first[q] + tk];
+        const uint64_t e = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(ev >> 32)) << 32) |
+                           (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)ev);
         const int32_t phase = (int32_t)((e >> 32) & 3u);
         const DJob& J = jobs[e >> 34];
         const int64_t t = (int64_t)(uint32_t)e;
