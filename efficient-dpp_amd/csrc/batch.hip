@@ -70,15 +70,15 @@ struct BatchShape {
     int block, scan_vpt32, scan_vpt64, split_ipt, gather_u;
 };
 constexpr BatchShape kShapes[] = {
-    {256, 8, 4, 8, 4},   // 0: scan 8192 / 2048, split 2048, gather 4096 (the default)
+    {256, 8, 4, 8, 4},   // 0: scan 8192 / 2048, split 2048, gather 4096
     {256, 4, 2, 4, 2},   // 1: half-size tiles: more tiles for small jobs
     {256, 8, 4, 8, 8},   // 2: gather tiles of 8192
     {512, 4, 2, 8, 4},   // 3: 8 waves, scan 8192 / 2048, split 4096, gather 8192
     {512, 8, 4, 4, 2},   // 4: 8 waves, scan 16384 / 4096, split 2048, gather 4096
-    {256, 16, 8, 8, 8},  // 5: scan 16384 / 4096, gather 8192
+    {256, 16, 8, 8, 8},  // 5: scan 16384 / 4096, split 2048, gather 8192 (the default)
 };
 constexpr int kNumShapes = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
-constexpr int kDefaultShape = 0;
+constexpr int kDefaultShape = 5;  // fastest on the 10 000-job mix (1.83 against 1.92 ms for shape 0)
 constexpr int kOccupancies[] = {2, 4, 8};
 constexpr int kNumOcc = 3;
 constexpr int kNumOrders = EDPP_BATCH_ORDER_XCD + 1;
@@ -265,6 +265,45 @@ __device__ void gather_tile(const DJob& J, int64_t t) {
     T* out = static_cast<T*>(J.out);
     const uint32_t n = (uint32_t)J.n;
     const int64_t base = t * (int64_t)BLOCK * 4 * U;
+    if (base + (int64_t)BLOCK * 4 * U <= J.n) {
+        // a whole tile: every index vector, then every random read, then the
+        // stores, so that each group of loads is in flight at once; four
+        // valid indices leave as one 16-byte store per 16 bytes of output
+        // (batch buffers are 16-byte aligned)
+        v4u32 x[U];
+        #pragma unroll
+        for (int u = 0; u < U; ++u)
+            x[u] = ld_stream(reinterpret_cast<const v4u32*>(J.idx + base + (int64_t)u * BLOCK * 4 + threadIdx.x * 4));
+        __builtin_amdgcn_sched_barrier(0);
+        T v[U][4];
+        #pragma unroll
+        for (int u = 0; u < U; ++u)
+            #pragma unroll
+            for (int k = 0; k < 4; ++k) v[u][k] = (uint32_t)x[u][k] < n ? in[(uint32_t)x[u][k]] : T(0);
+        __builtin_amdgcn_sched_barrier(0);
+        #pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t e0 = base + (int64_t)u * BLOCK * 4 + threadIdx.x * 4;
+            const bool all = (uint32_t)x[u][0] < n && (uint32_t)x[u][1] < n && (uint32_t)x[u][2] < n &&
+                             (uint32_t)x[u][3] < n;
+            if (all) {
+                typedef typename vec16<T>::type V;
+                constexpr int EPV = vec16<T>::n;
+                #pragma unroll
+                for (int h = 0; h < 4 / EPV; ++h) {
+                    V o;
+                    #pragma unroll
+                    for (int k = 0; k < EPV; ++k) o[k] = v[u][h * EPV + k];
+                    st_stream(o, reinterpret_cast<V*>(out + e0 + h * EPV));
+                }
+            } else {
+                #pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if ((uint32_t)x[u][k] < n) out[e0 + k] = v[u][k];
+            }
+        }
+        return;
+    }
     #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int64_t e0 = base + (int64_t)u * BLOCK * 4 + threadIdx.x * 4;

@@ -56,7 +56,7 @@ int edpp_batch_destroy(edpp_batch* b);
 int edpp_batch_set_occupancy(edpp_batch* b, int32_t wgs_per_cu);
 /* Tile shapes of the persistent launch.  Shape k fixes the lanes (waves) per
  * workgroup and the elements per tile of each kind: 32- and 64-bit scan
- * tiles, split tiles (lanes x keys per lane) and gather tiles.  Shape 0 is
+ * tiles, split tiles (lanes x keys per lane) and gather tiles.  Shape 5 is
  * the default.  Setting a shape re-plans the batch's work queue; results do
  * not depend on the shape or the occupancy. */
 int edpp_batch_num_shapes(void);
