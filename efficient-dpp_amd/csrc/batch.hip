@@ -156,12 +156,23 @@ __device__ void hist_scan_tile(const DJob& J, int64_t t, char* smem, uint32_t sp
     const int64_t hlen = ((int64_t)1 << J.lg) * J.tiles;
     const int64_t base = t * TS::TILE;
     TS ts;
+    // all 16 words are read at once; only those whose count has not arrived
+    // yet are polled again
+    uint64_t w[4][TS::EPV];
     #pragma unroll
     for (int j = 0; j < 4; ++j)
         #pragma unroll
         for (int k = 0; k < TS::EPV; ++k) {
             const int64_t e = base + (int64_t)j * TS::SUB + (int64_t)threadIdx.x * TS::EPV + k;
-            ts.v[j][k] = e < hlen ? wait_tagged(J.his + e, kTagCount, spin_limit, err) : 0u;
+            w[j][k] = e < hlen ? ld_relaxed(J.his + e) : tagged(kTagCount, 0u);
+        }
+    #pragma unroll
+    for (int j = 0; j < 4; ++j)
+        #pragma unroll
+        for (int k = 0; k < TS::EPV; ++k) {
+            const int64_t e = base + (int64_t)j * TS::SUB + (int64_t)threadIdx.x * TS::EPV + k;
+            ts.v[j][k] = (uint32_t)(w[j][k] >> 32) == kTagCount ? (uint32_t)w[j][k]
+                                                                 : wait_tagged(J.his + e, kTagCount, spin_limit, err);
         }
     ts.scan(sm);
     LookbackState32 st;
