@@ -21,7 +21,7 @@ def _outcome(ctx, edpp, run, exact):
 
 def test_watchdog_flags_timeouts_and_the_context_recovers(torch_gpu, ctx, edpp, orc):
     torch = torch_gpu
-    n = 1 << 24  # 512 look-back tiles
+    n = 1 << 24  # 256 to 512 look-back tiles
     a = orc.gen_i32(n, 99) & 15
     ref, _ = orc.scan_exclusive(a)
     da = torch.from_numpy(a).cuda()
@@ -36,11 +36,17 @@ def test_watchdog_flags_timeouts_and_the_context_recovers(torch_gpu, ctx, edpp, 
     def scan():
         ctx.scan(da, out, n, edpp.SCAN_LOOKBACK)
 
+    def scan_persist():
+        ctx.scan(da, out, n, edpp.SCAN_PERSIST)
+
     def scan_exact():
         return np.array_equal(out.cpu().numpy(), ref)
 
     def split():
         ctx.split(dk, kout, 256, dv, vout, st, m, edpp.SPLIT_WG)
+
+    def split_chained():
+        ctx.split(dk, kout, 256, dv, vout, st, m, edpp.SPLIT_CHAINED)
 
     def split_exact():
         return (np.array_equal(kout.cpu().numpy(), ko) and np.array_equal(vout.cpu().numpy(), vo)
@@ -51,12 +57,15 @@ def test_watchdog_flags_timeouts_and_the_context_recovers(torch_gpu, ctx, edpp, 
     try:
         for _ in range(3):
             outcomes.append(("scan", _outcome(ctx, edpp, scan, scan_exact)))
+            outcomes.append(("scan persistent", _outcome(ctx, edpp, scan_persist, scan_exact)))
             outcomes.append(("split", _outcome(ctx, edpp, split, split_exact)))
+            outcomes.append(("split chained", _outcome(ctx, edpp, split_chained, split_exact)))
     finally:
         ctx.set_watchdog()
     print("zero-poll watchdog outcomes:", outcomes)
     assert all(o != "wrong" for _, o in outcomes), outcomes  # nothing wrong goes unflagged
-    for run, exact in ((scan, scan_exact), (split, split_exact)):
+    for run, exact in ((scan, scan_exact), (scan_persist, scan_exact), (split, split_exact),
+                       (split_chained, split_exact)):
         run()
         ctx.sync()
         assert exact()
