@@ -318,24 +318,37 @@ __device__ __forceinline__ void rank_subtile(const K* __restrict__ kin, const in
                                              K (&key)[IPT], int32_t (&val)[IPT], uint32_t flip = 0) {
     const int lane = lane_id();
     load_subtile<K, IPT>(kin, vin, sbase, n, lg, shift, bk, key, val, flip);
+    // Each item's match mask is reduced at once to its rank among the lanes
+    // and its group size, packed as below | size << 16, so that 16 32-bit
+    // words stay live instead of 16 64-bit masks; an LDS-matched item is
+    // reduced kPipe items later, when its mask read has long arrived, so the
+    // wave keeps issuing the next items' LDS operations meanwhile.
+    constexpr int kPipe = 3;
+    uint32_t info[IPT];
     uint64_t m[IPT];
     #pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-        const bool ok = bk[j] != 0xFFFFFFFFu;
-        if (BALLOT) {
-            m[j] = match_bucket(bk[j], lg, ballot(ok));
-        } else {
-            // the lowest valid lane's bucket group comes from a ballot; the
-            // other lanes (none of them in that bucket) match through wmask
-            const uint32_t b = ok ? bk[j] : 0u;
-            uint32_t bf;
-            const uint64_t mf = first_bucket_group(b, ok, &bf);
-            const bool lds = ok && b != bf;
-            if (lds) __hip_atomic_fetch_or(wmask + b, 1ull << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-            const uint64_t ml = lds ? __hip_atomic_load(wmask + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) : 0ull;
-            if (lds) __hip_atomic_store(wmask + b, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-            m[j] = lds ? ml : (ok ? mf : 0ull);
+    for (int j = 0; j < IPT + kPipe; ++j) {
+        if (j < IPT) {
+            const bool ok = bk[j] != 0xFFFFFFFFu;
+            if (BALLOT) {
+                m[j] = match_bucket(bk[j], lg, ballot(ok));
+            } else {
+                // the lowest valid lane's bucket group comes from a ballot; the
+                // other lanes (none of them in that bucket) match through wmask
+                const uint32_t b = ok ? bk[j] : 0u;
+                uint32_t bf;
+                const uint64_t mf = first_bucket_group(b, ok, &bf);
+                const bool lds = ok && b != bf;
+                if (lds)
+                    __hip_atomic_fetch_or(wmask + b, 1ull << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                const uint64_t ml =
+                    lds ? __hip_atomic_load(wmask + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) : 0ull;
+                if (lds) __hip_atomic_store(wmask + b, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                m[j] = lds ? ml : (ok ? mf : 0ull);
+            }
         }
+        const int r = BALLOT ? j : j - kPipe;  // the item reduced now
+        if (r >= 0 && r < IPT) info[r] = count_below(m[r]) | ((uint32_t)__popcll(m[r]) << 16);
     }
     #pragma unroll
     for (int j = 0; j < IPT; ++j) {
@@ -343,10 +356,9 @@ __device__ __forceinline__ void rank_subtile(const K* __restrict__ kin, const in
         const uint32_t b = ok ? bk[j] : 0u;
         const uint32_t prior =
             ok ? __hip_atomic_load(whist_w + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) : 0u;
-        const uint32_t below = count_below(m[j]);
+        const uint32_t below = info[j] & 0xFFFFu;
         if (ok && below == 0)
-            __hip_atomic_fetch_add(whist_w + b, (uint32_t)__popcll(m[j]), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
+            __hip_atomic_fetch_add(whist_w + b, info[j] >> 16, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         rk[j] = prior + below;
     }
 }
