@@ -87,8 +87,10 @@ __host__ __device__ constexpr size_t fixed_lds_bytes(int B, int S) {
 // staged after the keys through one buffer (place_and_store's SEP), and the
 // register budget is held to two workgroups per CU (4 waves per SIMD), which
 // their LDS allows.  RO: kReorder*; the none and fixed variants run with
-// 256 lanes.
-template <class K, int IPT, bool KV, bool BALLOT, int BLOCK, int RO = kReorderVaried>
+// 256 lanes.  Keys and values are loaded through raw buffer descriptors
+// (no branch per load); BST: the varied variant also stores through them,
+// for outputs whose byte offsets fit 31 bits.
+template <class K, int IPT, bool KV, bool BALLOT, int BLOCK, int RO = kReorderVaried, bool BST = false>
 __global__ __launch_bounds__(BLOCK, (BLOCK > kSplitBlock ? 2 * BLOCK / (4 * kWave) : 1)) void k_split_scatter(const K* __restrict__ kin, K* __restrict__ kout,
                                                          const int32_t* __restrict__ vin, int32_t* __restrict__ vout,
                                                          SplitGeom g, const int32_t* __restrict__ scanned) {
@@ -124,8 +126,8 @@ __global__ __launch_bounds__(BLOCK, (BLOCK > kSplitBlock ? 2 * BLOCK / (4 * kWav
         uint32_t bk[IPT], rk[IPT];
         K key[IPT];
         int32_t val[IPT];
-        rank_subtile<K, IPT, BALLOT>(kin, KV ? vin : nullptr, s, tend, g.lg, g.shift, whist + w * B, wmask + w * B,
-                                     bk, rk, key, val, g.flip);
+        rank_subtile<K, IPT, BALLOT, true>(kin, KV ? vin : nullptr, s, tend, g.lg, g.shift, whist + w * B,
+                                           wmask + w * B, bk, rk, key, val, g.flip);
         __syncthreads();
         const int cnt = (int)std::min<int64_t>((int64_t)S, tend - s);
         if (RO == kReorderNone)
@@ -133,6 +135,10 @@ __global__ __launch_bounds__(BLOCK, (BLOCK > kSplitBlock ? 2 * BLOCK / (4 * kWav
         else if (RO == kReorderFixed)
             place_fixed<K, IPT, KV, BLOCK>(whist, run, sub, off, wtmp, kbuf, vbuf, ck, cv, ccnt, bk, rk, key, val, B,
                                            g.shift, cnt, kout, vout, g.n, g.flip);
+        else if constexpr (BST)
+            place_and_store_to<K, IPT, KV, BLOCK, SEP>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk, key, val, B,
+                                                       g.shift, cnt, buf_sink(kout, g.n), buf_sink(vout, g.n),
+                                                       g.flip, NoPlaceHook());
         else
             place_and_store<K, IPT, KV, BLOCK, SEP>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk, key, val, B,
                                                     g.shift, cnt, kout, vout, g.n, g.flip);
@@ -658,6 +664,23 @@ int run_wg(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
     if (a.d_start)
         hipLaunchKernelGGL(k_split_start, dim3((unsigned)ceil_div(B, 256)), dim3(256), 0, c->stream,
                            (const int32_t*)his, g.tiles, B, a.d_start);
+    if constexpr (RO == kReorderVaried) {
+        // buffer stores while every output offset fits the descriptor's 31 bits
+        if ((uint64_t)a.n * sizeof(K) < (1ull << 31)) {
+            EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_scatter<K, IPT, true, BALLOT, BLOCK, RO, true>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls));
+            EDPP_HIP(c, hipFuncSetAttribute((const void*)k_split_scatter<K, IPT, false, BALLOT, BLOCK, RO, true>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)ls));
+            if (a.vin)
+                hipLaunchKernelGGL((k_split_scatter<K, IPT, true, BALLOT, BLOCK, RO, true>), dim3((unsigned)g.tiles),
+                                   dim3(BLOCK), ls, c->stream, a.kin, a.kout, a.vin, a.vout, g, (const int32_t*)his);
+            else
+                hipLaunchKernelGGL((k_split_scatter<K, IPT, false, BALLOT, BLOCK, RO, true>), dim3((unsigned)g.tiles),
+                                   dim3(BLOCK), ls, c->stream, a.kin, a.kout, a.vin, a.vout, g, (const int32_t*)his);
+            EDPP_HIP(c, hipGetLastError());
+            return EDPP_OK;
+        }
+    }
     if (a.vin)
         hipLaunchKernelGGL((k_split_scatter<K, IPT, true, BALLOT, BLOCK, RO>), dim3((unsigned)g.tiles), dim3(BLOCK),
                            ls, c->stream, a.kin, a.kout, a.vin, a.vout, g, (const int32_t*)his);

@@ -85,12 +85,30 @@ __device__ __forceinline__ uint32_t bucket_of(K key, int shift, uint32_t mask, u
     return ((((uint32_t)key) >> shift) & mask) ^ flip;
 }
 
+// Raw buffer descriptor of n elements at p (gfx9 word 3: 32-bit data
+// format), and a load through one of the element at byte offset off, which
+// reads 0 past the descriptor's end (the memory unit checks the bound).
+template <class T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const T* p, int64_t n) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, 0, (int)(n * (int64_t)sizeof(T)), 0x00020000);
+}
+template <class T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff = 0) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, (int)soff, 0));
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, (int)soff, 0));
+    }
+}
+
 // Loads the calling wave's chunk of a sub-tile (item j of lane l is element
 // sbase + w*64*IPT + j*64 + l), and its values when vin is given, then
 // extracts buckets (0xFFFFFFFF past n).  Every load is issued before any is
 // used (the scheduling barrier keeps the compiler from consuming, and so
-// waiting for, each load before issuing the next).
-template <class K, int IPT>
+// waiting for, each load before issuing the next).  BUF: each wave loads
+// through raw buffer descriptors that start at its chunk and end at n, so
+// that a load past n reads 0 without a branch per load.
+template <class K, int IPT, bool BUF = false>
 __device__ __forceinline__ void load_subtile(const K* __restrict__ kin, const int32_t* __restrict__ vin, int64_t sbase,
                                              int64_t n, int lg, int shift, uint32_t (&bk)[IPT], K (&key)[IPT],
                                              int32_t (&val)[IPT], uint32_t flip = 0) {
@@ -98,16 +116,32 @@ __device__ __forceinline__ void load_subtile(const K* __restrict__ kin, const in
     const int w = threadIdx.x / kWave;
     const uint32_t mask = (1u << lg) - 1u;
     const int64_t wbase = sbase + (int64_t)w * kWave * IPT;
-    #pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-        const int64_t e = wbase + (int64_t)j * kWave + lane;
-        key[j] = e < n ? kin[e] : K(0);
-    }
-    if (vin) {
+    if constexpr (BUF) {
+        // the wave's chunk starts at a wave-uniform element, so the
+        // descriptors stay in scalar registers
+        const int64_t wb = sbase + (int64_t)__builtin_amdgcn_readfirstlane(w) * kWave * IPT;
+        const int64_t have = std::min<int64_t>(std::max<int64_t>(n - wb, 0), (int64_t)kWave * IPT);
+        const __amdgpu_buffer_rsrc_t rk = buf_rsrc(kin + wb, have);
+        #pragma unroll
+        for (int j = 0; j < IPT; ++j)
+            key[j] = buf_load<K>(rk, (uint32_t)lane * (uint32_t)sizeof(K), (uint32_t)(j * kWave * sizeof(K)));
+        if (vin) {
+            const __amdgpu_buffer_rsrc_t rv = buf_rsrc(vin + wb, have);
+            #pragma unroll
+            for (int j = 0; j < IPT; ++j) val[j] = buf_load<int32_t>(rv, (uint32_t)lane * 4u, (uint32_t)(j * kWave * 4));
+        }
+    } else {
         #pragma unroll
         for (int j = 0; j < IPT; ++j) {
             const int64_t e = wbase + (int64_t)j * kWave + lane;
-            val[j] = e < n ? vin[e] : 0;
+            key[j] = e < n ? kin[e] : K(0);
+        }
+        if (vin) {
+            #pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int64_t e = wbase + (int64_t)j * kWave + lane;
+                val[j] = e < n ? vin[e] : 0;
+            }
         }
     }
     __builtin_amdgcn_sched_barrier(0);
@@ -285,13 +319,18 @@ __device__ __forceinline__ void count_tiles(const K* __restrict__ kin, int64_t s
     const int B = 1 << lg;
     const int64_t wbase = sbase + (int64_t)w * kWave * IPT + lane;
     K key[TPW][IPT];
+    // one raw buffer descriptor from the wave's first element to n (loads
+    // past n read 0, with no branch per load; the wave-uniform start keeps it
+    // in scalar registers)
+    const int64_t wb = sbase + (int64_t)__builtin_amdgcn_readfirstlane(w) * kWave * IPT;
+    const __amdgpu_buffer_rsrc_t rk =
+        buf_rsrc(kin + wb, std::min<int64_t>(std::max<int64_t>(n - wb, 0), (int64_t)TPW * S));
     #pragma unroll
     for (int t = 0; t < TPW; ++t)
         #pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-            const int64_t e = wbase + (int64_t)t * S + (int64_t)j * kWave;
-            key[t][j] = e < n ? kin[e] : K(0);
-        }
+        for (int j = 0; j < IPT; ++j)
+            key[t][j] = buf_load<K>(rk, (uint32_t)lane * (uint32_t)sizeof(K),
+                                    (uint32_t)((t * S + j * kWave) * (int)sizeof(K)));
     __builtin_amdgcn_sched_barrier(0);
     #pragma unroll
     for (int t = 0; t < TPW; ++t)
@@ -311,13 +350,13 @@ __device__ __forceinline__ void count_tiles(const K* __restrict__ kin, int64_t s
 // neither phase waits for an earlier item's result before issuing the next
 // item: item j's mask read sees exactly item j's ORs (item j-1 cleared its
 // slots before them), and item j's whist_w read sees every add of items < j.
-template <class K, int IPT, bool BALLOT>
+template <class K, int IPT, bool BALLOT, bool BUF = false>
 __device__ __forceinline__ void rank_subtile(const K* __restrict__ kin, const int32_t* __restrict__ vin,
                                              int64_t sbase, int64_t n, int lg, int shift, uint32_t* whist_w,
                                              uint64_t* wmask, uint32_t (&bk)[IPT], uint32_t (&rk)[IPT],
                                              K (&key)[IPT], int32_t (&val)[IPT], uint32_t flip = 0) {
     const int lane = lane_id();
-    load_subtile<K, IPT>(kin, vin, sbase, n, lg, shift, bk, key, val, flip);
+    load_subtile<K, IPT, BUF>(kin, vin, sbase, n, lg, shift, bk, key, val, flip);
     // Each item's match mask is reduced at once to its rank among the lanes
     // and its group size, packed as below | size << 16, so that 16 32-bit
     // words stay live instead of 16 64-bit masks; an LDS-matched item is
@@ -414,13 +453,46 @@ struct NoPlaceHook {
     __device__ __forceinline__ void resolve() const {}
 };
 
-template <class K, int IPT, bool KV, int BLOCK = kSplitBlock, bool SEP = false, class Hook = NoPlaceHook>
-__device__ __forceinline__ void place_and_store(uint32_t* whist, uint32_t* run, uint32_t* sub, uint32_t* off,
-                                                uint32_t* wtmp, K* kbuf, int32_t* vbuf, const uint32_t (&bk)[IPT],
-                                                const uint32_t (&rk)[IPT], const K (&key)[IPT],
-                                                const int32_t (&val)[IPT], int B, int shift, int cnt,
-                                                K* __restrict__ kout, int32_t* __restrict__ vout, int64_t n,
-                                                uint32_t flip = 0, const Hook& hook = Hook()) {
+// Where place_and_store writes element i of its output (i < n; the scatter
+// never forms a larger i from a consistent histogram, and an empty slot's
+// 0xFFFFFFFF is dropped).  PtrSink tests the bound and stores through a
+// global pointer.  BufSink stores through a raw buffer descriptor of n
+// elements, whose bound the memory unit checks (an out-of-range store is
+// dropped), so no per-store branch is needed; it needs n * sizeof(T) < 2^32.
+template <class T>
+struct PtrSink {
+    T* __restrict__ p;
+    int64_t n;
+    __device__ __forceinline__ void put(uint32_t i, T v) const {
+        if ((int64_t)i < n) p[i] = v;
+    }
+};
+template <class T>
+struct BufSink {
+    __amdgpu_buffer_rsrc_t r;
+    uint32_t n;
+    __device__ __forceinline__ void put(uint32_t i, T v) const {
+        const int off = (int)((i < n ? i : n) * (uint32_t)sizeof(T));  // n: past the end, dropped
+        if constexpr (sizeof(T) == 4) {
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, 0);
+        } else {
+            typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, 0);
+        }
+    }
+};
+template <class T>
+__device__ __forceinline__ BufSink<T> buf_sink(T* p, int64_t n) {
+    return {buf_rsrc(p, n), (uint32_t)n};
+}
+
+template <class K, int IPT, bool KV, int BLOCK, bool SEP, class Hook, class KS, class VS>
+__device__ __forceinline__ void place_and_store_to(uint32_t* whist, uint32_t* run, uint32_t* sub, uint32_t* off,
+                                                   uint32_t* wtmp, K* kbuf, int32_t* vbuf,
+                                                   const uint32_t (&bk)[IPT], const uint32_t (&rk)[IPT],
+                                                   const K (&key)[IPT], const int32_t (&val)[IPT], int B,
+                                                   int shift, int cnt, const KS& kout, const VS& vout,
+                                                   uint32_t flip, const Hook& hook) {
     constexpr int W = BLOCK / kWave;
     constexpr int S = BLOCK * IPT;
     const int w = threadIdx.x / kWave;
@@ -472,7 +544,7 @@ __device__ __forceinline__ void place_and_store(uint32_t* whist, uint32_t* run, 
             const int i = threadIdx.x + j * BLOCK;
             const K k = kbuf[i < cnt ? i : 0];
             d[j] = i < cnt ? dlt[bucket_of(k, shift, mask, flip)] + (uint32_t)i : 0xFFFFFFFFu;
-            if ((int64_t)d[j] < n) kout[d[j]] = k;  // false only for empty slots
+            kout.put(d[j], k);  // dropped only for empty slots
         }
         __syncthreads();  // every key read before the values overwrite the buffer
         #pragma unroll
@@ -482,18 +554,28 @@ __device__ __forceinline__ void place_and_store(uint32_t* whist, uint32_t* run, 
         #pragma unroll
         for (int j = 0; j < IPT; ++j) {
             const int i = threadIdx.x + j * BLOCK;
-            if ((int64_t)d[j] < n) vout[d[j]] = vbuf[i];
+            vout.put(d[j], vbuf[i]);
         }
     } else {
         for (int i = threadIdx.x; i < cnt; i += BLOCK) {
             const K k = kbuf[i];
             const uint32_t p = dlt[bucket_of(k, shift, mask, flip)] + (uint32_t)i;
-            if ((int64_t)p < n) {  // always true for a consistent histogram scan
-                kout[p] = k;
-                if (KV) vout[p] = vbuf[i];
-            }
+            kout.put(p, k);
+            if constexpr (KV) vout.put(p, vbuf[i]);
         }
     }
+}
+
+template <class K, int IPT, bool KV, int BLOCK = kSplitBlock, bool SEP = false, class Hook = NoPlaceHook>
+__device__ __forceinline__ void place_and_store(uint32_t* whist, uint32_t* run, uint32_t* sub, uint32_t* off,
+                                                uint32_t* wtmp, K* kbuf, int32_t* vbuf, const uint32_t (&bk)[IPT],
+                                                const uint32_t (&rk)[IPT], const K (&key)[IPT],
+                                                const int32_t (&val)[IPT], int B, int shift, int cnt,
+                                                K* __restrict__ kout, int32_t* __restrict__ vout, int64_t n,
+                                                uint32_t flip = 0, const Hook& hook = Hook()) {
+    place_and_store_to<K, IPT, KV, BLOCK, SEP, Hook>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk, key, val, B,
+                                                     shift, cnt, PtrSink<K>{kout, n}, PtrSink<int32_t>{vout, n},
+                                                     flip, hook);
 }
 
 // ------------------------------------------------------------- reorder variants
