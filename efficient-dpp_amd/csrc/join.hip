@@ -7,8 +7,10 @@
 // obsolete/OpenCL/hj_partitioned_kernel.cl:7-115) and its count-only result
 // (phjImpl.cpp:13 "only count the number of outputs").  The count is
 //   sum over keys k of |{i: R[i] == k}| * |{j: S[j] == k}|.
-// Here the keys are first replaced by a bijective mix of their bits, so the
-// partitions stay balanced for structured keys; equal keys keep equal mixes.
+// Here the keys are replaced by a bijective mix of their bits (mix32), so that
+// the partitions stay balanced for structured keys; equal keys keep equal
+// mixes.  The split's first pass mixes each key as it loads it (MixOnLoad),
+// so the mixed keys are never written and read back by a pass of their own.
 // The partition width targets 4096 build keys (up to 16 partition bits); a
 // workgroup builds its table from the smaller side of its pair in chunks of at
 // most kJoinChunk keys (load factor <= 1/2), so any skew ends, at the cost of
@@ -17,6 +19,7 @@
 #include <algorithm>
 #include "ctx.hpp"
 #include "device.hpp"
+#include "split_tile.hpp"
 
 namespace edpp {
 
@@ -31,21 +34,6 @@ constexpr int kJoinPre = kJoinChunk / kJoinBlock;  // keys per lane per side hel
 // passes, so relations up to 2^28 keys keep about kJoinTargetBuild build keys
 // per partition and one table fill each.
 constexpr int kJoinMaxBits = 16;
-
-// murmur3's 32-bit finalizer: a bijection of 32-bit words
-__host__ __device__ inline uint32_t mix32(uint32_t x) {
-    x ^= x >> 16;
-    x *= 0x85EBCA6Bu;
-    x ^= x >> 13;
-    x *= 0xC2B2AE35u;
-    x ^= x >> 16;
-    return x;
-}
-
-__global__ void k_join_mix(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = mix32(in[i]);
-}
 
 __global__ void k_join_zero(unsigned long long* p) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0ull;
@@ -88,9 +76,9 @@ __device__ __forceinline__ uint32_t join_lookup(const uint64_t* table, uint32_t 
 }
 
 // One workgroup per partition pair b: R rows [rs[b], rs[b+1]) and S rows
-// [ss[b], ss[b+1]) of the partitioned mixed keys.  A table fill holds at most
-// kJoinChunk distinct keys in kJoinSlots slots, so every probe sequence meets
-// an empty slot and ends.
+// [ss[b], ss[b+1]) of the partitioned mixed keys.
+// A table fill holds at most kJoinChunk distinct keys in kJoinSlots slots, so
+// every probe sequence meets an empty slot and ends.
 __global__ __launch_bounds__(kJoinBlock) void k_join_probe(const uint32_t* __restrict__ rk, int64_t rn,
                                                            const int32_t* __restrict__ rs,
                                                            const uint32_t* __restrict__ sk, int64_t sn,
@@ -171,22 +159,19 @@ int launch_join_count(edpp_ctx* c, const uint32_t* rk, int64_t rn, const uint32_
     const size_t rb = ((size_t)rn * 4 + 255) & ~(size_t)255, sb = ((size_t)sn * 4 + 255) & ~(size_t)255;
     const size_t stb = ((size_t)B * 4 + 255) & ~(size_t)255;
     void* w;
-    int rc = ws_outer_reserve(c, 2 * rb + 2 * sb + 2 * stb, &w);
+    int rc = ws_outer_reserve(c, rb + sb + 2 * stb, &w);
     if (rc) return rc;
     char* base = static_cast<char*>(w);
-    uint32_t* rmix = reinterpret_cast<uint32_t*>(base);
-    uint32_t* rpart = reinterpret_cast<uint32_t*>(base + rb);
-    uint32_t* smix = reinterpret_cast<uint32_t*>(base + 2 * rb);
-    uint32_t* spart = reinterpret_cast<uint32_t*>(base + 2 * rb + sb);
-    int32_t* rst = reinterpret_cast<int32_t*>(base + 2 * rb + 2 * sb);
-    int32_t* sst = reinterpret_cast<int32_t*>(base + 2 * rb + 2 * sb + stb);
-    const unsigned grid_r = (unsigned)std::min<int64_t>(ceil_div(rn, 256), (int64_t)c->cus * 8);
-    const unsigned grid_s = (unsigned)std::min<int64_t>(ceil_div(sn, 256), (int64_t)c->cus * 8);
-    hipLaunchKernelGGL(k_join_mix, dim3(grid_r), dim3(256), 0, c->stream, rk, rmix, rn);
-    hipLaunchKernelGGL(k_join_mix, dim3(grid_s), dim3(256), 0, c->stream, sk, smix, sn);
-    EDPP_HIP(c, hipGetLastError());
-    rc = launch_split<uint32_t>(c, rmix, rpart, nullptr, nullptr, rst, rn, B, EDPP_SPLIT_AUTO);
-    if (!rc) rc = launch_split<uint32_t>(c, smix, spart, nullptr, nullptr, sst, sn, B, EDPP_SPLIT_AUTO);
+    uint32_t* rpart = reinterpret_cast<uint32_t*>(base);
+    uint32_t* spart = reinterpret_cast<uint32_t*>(base + rb);
+    int32_t* rst = reinterpret_cast<int32_t*>(base + rb + sb);
+    int32_t* sst = reinterpret_cast<int32_t*>(base + rb + sb + stb);
+    // the partitioned relations hold the mixed keys
+    rc = launch_split<MixOnLoad>(c, reinterpret_cast<const MixOnLoad*>(rk), reinterpret_cast<MixOnLoad*>(rpart),
+                                 nullptr, nullptr, rst, rn, B, EDPP_SPLIT_AUTO);
+    if (!rc)
+        rc = launch_split<MixOnLoad>(c, reinterpret_cast<const MixOnLoad*>(sk), reinterpret_cast<MixOnLoad*>(spart),
+                                     nullptr, nullptr, sst, sn, B, EDPP_SPLIT_AUTO);
     if (rc) return rc;
     const size_t lds = (size_t)kJoinSlots * 8;
     EDPP_HIP(c, hipFuncSetAttribute((const void*)k_join_probe, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -17,6 +17,7 @@
 //           from registers (WI_split / the CUDA warp multisplit).
 //   SINGLE  one lane walks a contiguous chunk (single_split).
 #include <algorithm>
+#include <type_traits>
 #include "ctx.hpp"
 #include "split_tile.hpp"
 #include "tile_scan.hpp"
@@ -532,7 +533,7 @@ __global__ void k_split_start_search(const K* __restrict__ kout, int64_t n, int 
         int64_t lo = 0, hi = n;
         while (lo < hi) {
             const int64_t mid = lo + (hi - lo) / 2;
-            if ((((uint32_t)kout[mid]) & mask) < (uint32_t)b) lo = mid + 1;
+            if (bucket_of(kout[mid], 0, mask, 0u) < (uint32_t)b) lo = mid + 1;
             else hi = mid;
         }
         d_start[b] = (int32_t)lo;
@@ -751,38 +752,50 @@ int run_single(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
 
 template <class K>
 int run_pass(edpp_ctx* c, int strategy, const SplitCall<K>& a, int32_t* his) {
-    switch (strategy) {
-        case EDPP_SPLIT_WG: {
-            const bool bal = a.lg <= kBallotMatchBits;
-            const int ro = c->tune.split_reorder;
-            if (ro == EDPP_REORDER_NONE || ro == EDPP_REORDER_FIXED) {
-                // 256 lanes (wg_block); wg_ipt also checks that the fixed
-                // variant's staging fits beside 16 keys per lane
-                const bool fx = ro == EDPP_REORDER_FIXED;
-                const bool i16 = wg_ipt<K>(c, a.lg) == 16;
-                if (fx) {
-                    if (i16) return bal ? run_wg<K, 16, true, kSplitBlock, kReorderFixed>(c, a, his)
-                                        : run_wg<K, 16, false, kSplitBlock, kReorderFixed>(c, a, his);
-                    return bal ? run_wg<K, 8, true, kSplitBlock, kReorderFixed>(c, a, his)
-                               : run_wg<K, 8, false, kSplitBlock, kReorderFixed>(c, a, his);
-                }
-                if (i16) return bal ? run_wg<K, 16, true, kSplitBlock, kReorderNone>(c, a, his)
-                                    : run_wg<K, 16, false, kSplitBlock, kReorderNone>(c, a, his);
-                return bal ? run_wg<K, 8, true, kSplitBlock, kReorderNone>(c, a, his)
-                           : run_wg<K, 8, false, kSplitBlock, kReorderNone>(c, a, his);
-            }
-            if (wg_ipt<K>(c, a.lg) == 16) {
-                if constexpr (sizeof(K) == 4) {
-                    if (wg_block<K>(c, a.lg) == 512)
-                        return bal ? run_wg<K, 16, true, 512>(c, a, his) : run_wg<K, 16, false, 512>(c, a, his);
-                }
-                return bal ? run_wg<K, 16, true, kSplitBlock>(c, a, his) : run_wg<K, 16, false, kSplitBlock>(c, a, his);
-            }
-            return bal ? run_wg<K, 8, true, kSplitBlock>(c, a, his) : run_wg<K, 8, false, kSplitBlock>(c, a, his);
+    if constexpr (std::is_same_v<K, MixOnLoad>) {
+        // the hash join's partition: the WG strategy with the varied reorder
+        (void)strategy;
+        const bool bal = a.lg <= kBallotMatchBits;
+        if (wg_ipt<K>(c, a.lg) == 16) {
+            if (wg_block<K>(c, a.lg) == 512)
+                return bal ? run_wg<K, 16, true, 512>(c, a, his) : run_wg<K, 16, false, 512>(c, a, his);
+            return bal ? run_wg<K, 16, true, kSplitBlock>(c, a, his) : run_wg<K, 16, false, kSplitBlock>(c, a, his);
         }
-        case EDPP_SPLIT_WI:
-            return a.lg <= kBallotMatchBits ? run_wi<K, 16, true>(c, a, his) : run_wi<K, 16, false>(c, a, his);
-        default: return run_single<K>(c, a, his);
+        return bal ? run_wg<K, 8, true, kSplitBlock>(c, a, his) : run_wg<K, 8, false, kSplitBlock>(c, a, his);
+    } else {
+        switch (strategy) {
+            case EDPP_SPLIT_WG: {
+                const bool bal = a.lg <= kBallotMatchBits;
+                const int ro = c->tune.split_reorder;
+                if (ro == EDPP_REORDER_NONE || ro == EDPP_REORDER_FIXED) {
+                    // 256 lanes (wg_block); wg_ipt also checks that the fixed
+                    // variant's staging fits beside 16 keys per lane
+                    const bool fx = ro == EDPP_REORDER_FIXED;
+                    const bool i16 = wg_ipt<K>(c, a.lg) == 16;
+                    if (fx) {
+                        if (i16) return bal ? run_wg<K, 16, true, kSplitBlock, kReorderFixed>(c, a, his)
+                                            : run_wg<K, 16, false, kSplitBlock, kReorderFixed>(c, a, his);
+                        return bal ? run_wg<K, 8, true, kSplitBlock, kReorderFixed>(c, a, his)
+                                   : run_wg<K, 8, false, kSplitBlock, kReorderFixed>(c, a, his);
+                    }
+                    if (i16) return bal ? run_wg<K, 16, true, kSplitBlock, kReorderNone>(c, a, his)
+                                        : run_wg<K, 16, false, kSplitBlock, kReorderNone>(c, a, his);
+                    return bal ? run_wg<K, 8, true, kSplitBlock, kReorderNone>(c, a, his)
+                               : run_wg<K, 8, false, kSplitBlock, kReorderNone>(c, a, his);
+                }
+                if (wg_ipt<K>(c, a.lg) == 16) {
+                    if constexpr (sizeof(K) == 4) {
+                        if (wg_block<K>(c, a.lg) == 512)
+                            return bal ? run_wg<K, 16, true, 512>(c, a, his) : run_wg<K, 16, false, 512>(c, a, his);
+                    }
+                    return bal ? run_wg<K, 16, true, kSplitBlock>(c, a, his) : run_wg<K, 16, false, kSplitBlock>(c, a, his);
+                }
+                return bal ? run_wg<K, 8, true, kSplitBlock>(c, a, his) : run_wg<K, 8, false, kSplitBlock>(c, a, his);
+            }
+            case EDPP_SPLIT_WI:
+                return a.lg <= kBallotMatchBits ? run_wi<K, 16, true>(c, a, his) : run_wi<K, 16, false>(c, a, his);
+            default: return run_single<K>(c, a, his);
+        }
     }
 }
 
@@ -931,7 +944,9 @@ int launch_split(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t
         return fail(c, EDPP_EINVAL, "unknown split strategy %d", strategy);
     const bool two = strategy != EDPP_SPLIT_SINGLE && lg > kMaxSplitBitsPerPass;
     const int lg1 = two ? lg / 2 : lg, lg2 = lg - lg1;
-    if (strategy == EDPP_SPLIT_CHAINED) {
+    if constexpr (std::is_same_v<K, MixOnLoad>) {
+        if (strategy != EDPP_SPLIT_WG) return fail(c, EDPP_EINVAL, "mix-on-load splits run the WG strategy");
+    } else if (strategy == EDPP_SPLIT_CHAINED) {
         // both digits of a two-pass split are counted by the one up-front read
         const size_t kbytes = two ? (((size_t)n * sizeof(K) + 255) & ~(size_t)255) : 0;
         const size_t vbytes = (two && vin) ? (((size_t)n * 4 + 255) & ~(size_t)255) : 0;
@@ -976,12 +991,16 @@ int launch_split(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t
     SplitCall<K> a1 = {kin, tk, vin, tv, nullptr, n, lg1, 0, 0u};
     rc = run_pass<K>(c, strategy, a1, his);
     if (rc) return rc;
-    SplitCall<K> a2 = {tk, kout, tv, vout, nullptr, n, lg2, lg1, 0u};
-    rc = run_pass<K>(c, strategy, a2, his);
+    // the second pass reads what the first wrote: for MixOnLoad keys, the
+    // mixed keys themselves, split as plain 32-bit keys
+    using K2 = std::conditional_t<std::is_same_v<K, MixOnLoad>, uint32_t, K>;
+    K2* kout2 = reinterpret_cast<K2*>(kout);
+    SplitCall<K2> a2 = {reinterpret_cast<const K2*>(tk), kout2, tv, vout, nullptr, n, lg2, lg1, 0u};
+    rc = run_pass<K2>(c, strategy, a2, his);
     if (rc) return rc;
     if (d_start) {
-        hipLaunchKernelGGL((k_split_start_search<K>), dim3((unsigned)ceil_div(buckets, 256)), dim3(256), 0, c->stream,
-                           kout, n, lg, d_start);
+        hipLaunchKernelGGL((k_split_start_search<K2>), dim3((unsigned)ceil_div(buckets, 256)), dim3(256), 0, c->stream,
+                           kout2, n, lg, d_start);
         EDPP_HIP(c, hipGetLastError());
     }
     return EDPP_OK;
@@ -1082,6 +1101,8 @@ int launch_sort_u32(edpp_ctx* c, const uint32_t* kin, uint32_t* kout, const int3
     return rc;
 }
 
+template int launch_split<MixOnLoad>(edpp_ctx*, const MixOnLoad*, MixOnLoad*, const int32_t*, int32_t*, int32_t*,
+                                     int64_t, int, int);
 template int launch_split<uint32_t>(edpp_ctx*, const uint32_t*, uint32_t*, const int32_t*, int32_t*, int32_t*, int64_t,
                                     int, int);
 template int launch_split<uint64_t>(edpp_ctx*, const uint64_t*, uint64_t*, const int32_t*, int32_t*, int32_t*,

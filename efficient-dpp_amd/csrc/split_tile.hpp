@@ -85,6 +85,37 @@ __device__ __forceinline__ uint32_t bucket_of(K key, int shift, uint32_t mask, u
     return ((((uint32_t)key) >> shift) & mask) ^ flip;
 }
 
+// murmur3's 32-bit finalizer: a bijection of 32-bit words
+__host__ __device__ inline uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x85EBCA6Bu;
+    x ^= x >> 13;
+    x *= 0xC2B2AE35u;
+    x ^= x >> 16;
+    return x;
+}
+
+// A 32-bit key replaced by mix32(key) as it is loaded: a split of MixOnLoad
+// keys is the split of the mixed keys, read straight from the unmixed ones.
+// The hash join partitions its relations so, which keeps structured keys
+// spread evenly over the partitions without a separate pass that mixes them.
+struct MixOnLoad {
+    uint32_t v;
+    MixOnLoad() = default;
+    __host__ __device__ constexpr explicit MixOnLoad(uint32_t x) : v(x) {}
+};
+__device__ __forceinline__ uint32_t bucket_of(MixOnLoad key, int shift, uint32_t mask, uint32_t flip) {
+    return ((key.v >> shift) & mask) ^ flip;
+}
+// A key as the split sees it once loaded: itself, or mixed for MixOnLoad.
+template <class K>
+__device__ __forceinline__ K on_load(K k) {
+    return k;
+}
+__device__ __forceinline__ MixOnLoad on_load(MixOnLoad k) {
+    return MixOnLoad(mix32(k.v));
+}
+
 // Raw buffer descriptor of n elements at p (gfx9 word 3: 32-bit data
 // format), and a load through one of the element at byte offset off, which
 // reads 0 past the descriptor's end (the memory unit checks the bound).
@@ -124,7 +155,7 @@ __device__ __forceinline__ void load_subtile(const K* __restrict__ kin, const in
         const __amdgpu_buffer_rsrc_t rk = buf_rsrc(kin + wb, have);
         #pragma unroll
         for (int j = 0; j < IPT; ++j)
-            key[j] = buf_load<K>(rk, (uint32_t)lane * (uint32_t)sizeof(K), (uint32_t)(j * kWave * sizeof(K)));
+            key[j] = on_load(buf_load<K>(rk, (uint32_t)lane * (uint32_t)sizeof(K), (uint32_t)(j * kWave * sizeof(K))));
         if (vin) {
             const __amdgpu_buffer_rsrc_t rv = buf_rsrc(vin + wb, have);
             #pragma unroll
@@ -134,7 +165,7 @@ __device__ __forceinline__ void load_subtile(const K* __restrict__ kin, const in
         #pragma unroll
         for (int j = 0; j < IPT; ++j) {
             const int64_t e = wbase + (int64_t)j * kWave + lane;
-            key[j] = e < n ? kin[e] : K(0);
+            key[j] = e < n ? on_load(kin[e]) : K(0);
         }
         if (vin) {
             #pragma unroll
@@ -329,8 +360,8 @@ __device__ __forceinline__ void count_tiles(const K* __restrict__ kin, int64_t s
     for (int t = 0; t < TPW; ++t)
         #pragma unroll
         for (int j = 0; j < IPT; ++j)
-            key[t][j] = buf_load<K>(rk, (uint32_t)lane * (uint32_t)sizeof(K),
-                                    (uint32_t)((t * S + j * kWave) * (int)sizeof(K)));
+            key[t][j] = on_load(buf_load<K>(rk, (uint32_t)lane * (uint32_t)sizeof(K),
+                                            (uint32_t)((t * S + j * kWave) * (int)sizeof(K))));
     __builtin_amdgcn_sched_barrier(0);
     #pragma unroll
     for (int t = 0; t < TPW; ++t)
