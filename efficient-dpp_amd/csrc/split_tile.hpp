@@ -489,7 +489,8 @@ struct NoPlaceHook {
 // 0xFFFFFFFF is dropped).  PtrSink tests the bound and stores through a
 // global pointer.  BufSink stores through a raw buffer descriptor of n
 // elements, whose bound the memory unit checks (an out-of-range store is
-// dropped), so no per-store branch is needed; it needs n * sizeof(T) < 2^32.
+// dropped), so no per-store branch is needed; it needs n * sizeof(T) < 2^31,
+// since byte offsets travel as signed 32-bit operands.
 template <class T>
 struct PtrSink {
     T* __restrict__ p;
