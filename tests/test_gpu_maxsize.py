@@ -82,3 +82,27 @@ def test_sort_max_n(big, ctx):
     ctx.gen_i32(ref, 15)
     assert torch.equal(ref[vals.long()], keys)
     del keys, vals, ref
+
+
+@pytest.mark.parametrize("wide,n", [(False, (1 << 29) - 1), (False, 1 << 29), (True, (1 << 28) - 1), (True, 1 << 28)])
+def test_split_at_the_buffer_store_limit(big, ctx, wide, n):
+    """The WG scatter stores through buffer descriptors while the output's
+    byte offsets fit 31 bits (n * key bytes < 2^31) and through global
+    pointers from there on: both sides of that switch, 256 buckets."""
+    torch = big
+    dt = torch.int64 if wide else torch.int32
+    keys = torch.randint(-2**31, 2**31 - 1, (n,), dtype=dt, device="cuda")
+    vals = torch.arange(n, dtype=torch.int32, device="cuda")
+    kout = torch.empty_like(keys)
+    vout = torch.empty_like(vals)
+    st = torch.empty(256, dtype=torch.int32, device="cuda")
+    ctx.split(keys, kout, 256, vals, vout, st, n, 1, wide=wide)
+    b = kout & 255
+    assert bool((b[1:] >= b[:-1]).all())
+    same = b[1:] == b[:-1]
+    assert bool((vout[1:][same] > vout[:-1][same]).all())  # stable
+    del same, b
+    assert torch.equal(keys[vout.long()], kout)  # a permutation of the pairs
+    cnt = torch.bincount((keys & 255).long(), minlength=256)
+    assert torch.equal(st.long(), torch.cumsum(cnt, 0) - cnt)
+    del keys, vals, kout, vout
