@@ -84,6 +84,19 @@ __host__ __device__ constexpr size_t fixed_lds_bytes(int B, int S) {
     return (size_t)fixed_ck_word_offset<K>(B, S) * 4 + (size_t)B * kFixedCap * (sizeof(K) + 4);
 }
 
+// place_and_store hook of the scatter whose values follow its keys through
+// LDS: the sub-tile's values are loaded once its keys are placed.
+template <int IPT>
+struct LateValues {
+    static constexpr bool kSync = false;
+    int32_t (&val)[IPT];
+    const int32_t* __restrict__ vin;
+    int64_t s, n;
+    __device__ __forceinline__ void publish() const {}
+    __device__ __forceinline__ void resolve() const {}
+    __device__ __forceinline__ void keys_placed() const { load_chunk_buf<int32_t, IPT>(vin, s, n, val); }
+};
+
 // BLOCK threads; sub-tile S = BLOCK*IPT.  Above 256 threads the values are
 // staged after the keys through one buffer (place_and_store's SEP), and the
 // register budget is held to two workgroups per CU (4 waves per SIMD), which
@@ -127,8 +140,10 @@ __global__ __launch_bounds__(BLOCK, (BLOCK > kSplitBlock ? 2 * BLOCK / (4 * kWav
         uint32_t bk[IPT], rk[IPT];
         K key[IPT];
         int32_t val[IPT];
-        rank_subtile<K, IPT, BALLOT, true>(kin, KV ? vin : nullptr, s, tend, g.lg, g.shift, whist + w * B,
-                                           wmask + w * B, bk, rk, key, val, g.flip);
+        // values staged after the keys are loaded once the keys are in LDS
+        // (LateValues), so they are not held through the ranking
+        rank_subtile<K, IPT, BALLOT, true>(kin, KV && !(SEP && BST) ? vin : nullptr, s, tend, g.lg, g.shift,
+                                           whist + w * B, wmask + w * B, bk, rk, key, val, g.flip);
         __syncthreads();
         const int cnt = (int)std::min<int64_t>((int64_t)S, tend - s);
         if (RO == kReorderNone)
@@ -136,6 +151,10 @@ __global__ __launch_bounds__(BLOCK, (BLOCK > kSplitBlock ? 2 * BLOCK / (4 * kWav
         else if (RO == kReorderFixed)
             place_fixed<K, IPT, KV, BLOCK>(whist, run, sub, off, wtmp, kbuf, vbuf, ck, cv, ccnt, bk, rk, key, val, B,
                                            g.shift, cnt, kout, vout, g.n, g.flip);
+        else if constexpr (BST && SEP)
+            place_and_store_to<K, IPT, KV, BLOCK, SEP>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk, key, val, B,
+                                                       g.shift, cnt, buf_sink(kout, g.n), buf_sink(vout, g.n),
+                                                       g.flip, LateValues<IPT>{val, vin, s, tend});
         else if constexpr (BST)
             place_and_store_to<K, IPT, KV, BLOCK, SEP>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk, key, val, B,
                                                        g.shift, cnt, buf_sink(kout, g.n), buf_sink(vout, g.n),
@@ -308,6 +327,7 @@ struct ChainHook {
     int range, B, L;
     const uint32_t* sub;
     uint32_t* run;
+    __device__ __forceinline__ void keys_placed() const {}
     __device__ __forceinline__ void publish() const {
         const int b = threadIdx.x;
         if (b < B) st.publish(tile * B + b, tile == first ? kFlagPrefix : kFlagAggregate, sub[b]);

@@ -132,6 +132,21 @@ __device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, ui
     }
 }
 
+// Buffer loads of the calling wave's chunk of a sub-tile (item j of lane l is
+// element sbase + w*64*IPT + j*64 + l) through one descriptor from the chunk's
+// start to n: past n they read 0, with no branch per load, and the lane's
+// loads share one address register with scalar offsets.  The wave index is
+// read as wave-uniform, so the descriptor stays in scalar registers.
+template <class T, int IPT>
+__device__ __forceinline__ void load_chunk_buf(const T* __restrict__ p, int64_t sbase, int64_t n, T (&x)[IPT]) {
+    const int64_t wb = sbase + (int64_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave)) * kWave * IPT;
+    const __amdgpu_buffer_rsrc_t r =
+        buf_rsrc(p + wb, std::min<int64_t>(std::max<int64_t>(n - wb, 0), (int64_t)kWave * IPT));
+    #pragma unroll
+    for (int j = 0; j < IPT; ++j)
+        x[j] = buf_load<T>(r, (uint32_t)lane_id() * (uint32_t)sizeof(T), (uint32_t)(j * kWave * (int)sizeof(T)));
+}
+
 // Loads the calling wave's chunk of a sub-tile (item j of lane l is element
 // sbase + w*64*IPT + j*64 + l), and its values when vin is given, then
 // extracts buckets (0xFFFFFFFF past n).  Every load is issued before any is
@@ -148,19 +163,10 @@ __device__ __forceinline__ void load_subtile(const K* __restrict__ kin, const in
     const uint32_t mask = (1u << lg) - 1u;
     const int64_t wbase = sbase + (int64_t)w * kWave * IPT;
     if constexpr (BUF) {
-        // the wave's chunk starts at a wave-uniform element, so the
-        // descriptors stay in scalar registers
-        const int64_t wb = sbase + (int64_t)__builtin_amdgcn_readfirstlane(w) * kWave * IPT;
-        const int64_t have = std::min<int64_t>(std::max<int64_t>(n - wb, 0), (int64_t)kWave * IPT);
-        const __amdgpu_buffer_rsrc_t rk = buf_rsrc(kin + wb, have);
+        load_chunk_buf<K, IPT>(kin, sbase, n, key);
         #pragma unroll
-        for (int j = 0; j < IPT; ++j)
-            key[j] = on_load(buf_load<K>(rk, (uint32_t)lane * (uint32_t)sizeof(K), (uint32_t)(j * kWave * sizeof(K))));
-        if (vin) {
-            const __amdgpu_buffer_rsrc_t rv = buf_rsrc(vin + wb, have);
-            #pragma unroll
-            for (int j = 0; j < IPT; ++j) val[j] = buf_load<int32_t>(rv, (uint32_t)lane * 4u, (uint32_t)(j * kWave * 4));
-        }
+        for (int j = 0; j < IPT; ++j) key[j] = on_load(key[j]);
+        if (vin) load_chunk_buf<int32_t, IPT>(vin, sbase, n, val);
     } else {
         #pragma unroll
         for (int j = 0; j < IPT; ++j) {
@@ -482,6 +488,7 @@ struct NoPlaceHook {
     static constexpr bool kSync = false;  // resolve() writes nothing another thread reads
     __device__ __forceinline__ void publish() const {}
     __device__ __forceinline__ void resolve() const {}
+    __device__ __forceinline__ void keys_placed() const {}
 };
 
 // Where place_and_store writes element i of its output (i < n; the scatter
@@ -557,6 +564,7 @@ __device__ __forceinline__ void place_and_store_to(uint32_t* whist, uint32_t* ru
             if (KV && !SEP) vbuf[pos[j]] = val[j];
         }
     }
+    hook.keys_placed();
     // each thread owns its buckets here: nobody reads sub[] or run[] again
     // before the barrier below
     hook.resolve();
