@@ -198,7 +198,7 @@ int default_split_strategy(edpp_ctx* c, int64_t n, int buckets) {
 // Keys per lane of the WG split, from an autotuner sweep of 2^16..2^28
 // pairs: 8 below 2^22 (more workgroups while the call is latency-bound),
 // and for up to 16 buckets up to 2^24 and for two-pass bucket counts up to
-// 2^26; 16 otherwise (from 256 buckets the 512-lane scatter needs it: B=256
+// 2^26; 16 otherwise (from 128 buckets the 512-lane scatter needs it: B=256
 // at 2^28 is 1.33 ms at 16 against 2.31 ms at 8).
 void default_split_shape(edpp_ctx* c, int64_t n, int buckets, int key_bytes, int* ipt, int* mult) {
     (void)c;

@@ -616,16 +616,16 @@ int wg_ipt(edpp_ctx* c, int lg) {
 }
 
 // Lanes per scatter workgroup.  512 (sub-tiles of 8192 keys, values staged
-// after keys) from 256 buckets on: a bucket's run per sub-tile doubles, so
+// after keys) from 128 buckets on: a bucket's run per sub-tile doubles, so
 // fewer stores are partial cache lines; only for 4-byte keys at 16 per lane.
-// Up to 128 buckets the 256-lane workgroup is faster (B=128: 1.20 against
-// 1.22 ms uniform, 1.16 against 1.26 ms Zipf, 2^28 pairs).
+// B=128, 2^28 pairs: 1.12 against 1.22 ms uniform, 1.13 against 1.16 ms
+// Zipf.  Up to 64 buckets the 256-lane workgroup is as fast or faster.
 template <class K>
 int wg_block(edpp_ctx* c, int lg) {
     if (reorder_256(c)) return kSplitBlock;
     if (sizeof(K) != 4 || wg_ipt<K>(c, lg) != 16 || wg_lds_bytes<16, K, 512>(1 << lg) > 160 * 1024) return kSplitBlock;
     if (c->tune.split_block != 0) return c->tune.split_block;
-    return lg >= 8 ? 512 : kSplitBlock;
+    return lg >= 7 ? 512 : kSplitBlock;
 }
 
 // histogram entries one pass needs
