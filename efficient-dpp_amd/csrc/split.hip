@@ -607,8 +607,17 @@ bool reorder_256(edpp_ctx* c) {
     return c->tune.split_reorder == EDPP_REORDER_NONE || c->tune.split_reorder == EDPP_REORDER_FIXED;
 }
 
+// 8-byte keys from 128 buckets at the large-input shape: 512-lane scatter
+// workgroups with 8 keys per lane (the 4096-pair sub-tile of 256 lanes at 16,
+// with values staged after keys and loaded late)
 template <class K>
-int wg_ipt(edpp_ctx* c, int lg) {
+bool wide_u64(edpp_ctx* c, int lg) {
+    return sizeof(K) == 8 && !reorder_256(c) && c->tune.split_ipt >= 16 && c->tune.split_block != kSplitBlock &&
+           lg >= 7 && wg_lds_bytes<8, K, 512>(1 << lg) <= 160 * 1024;
+}
+
+template <class K>
+int wg_ipt_base(edpp_ctx* c, int lg) {
     if (c->tune.split_ipt < 16 || wg_lds_bytes<16, K>(1 << lg) > 160 * 1024) return 8;
     if (c->tune.split_reorder == EDPP_REORDER_FIXED && fixed_lds_bytes<K>(1 << lg, kSplitBlock * 16) > 160 * 1024)
         return 8;
@@ -621,11 +630,21 @@ int wg_ipt(edpp_ctx* c, int lg) {
 // B=128, 2^28 pairs: 1.12 against 1.22 ms uniform, 1.13 against 1.16 ms
 // Zipf.  Up to 64 buckets the 256-lane workgroup is as fast or faster.
 template <class K>
-int wg_block(edpp_ctx* c, int lg) {
+int wg_block_base(edpp_ctx* c, int lg) {
     if (reorder_256(c)) return kSplitBlock;
-    if (sizeof(K) != 4 || wg_ipt<K>(c, lg) != 16 || wg_lds_bytes<16, K, 512>(1 << lg) > 160 * 1024) return kSplitBlock;
+    if (sizeof(K) != 4 || wg_ipt_base<K>(c, lg) != 16 || wg_lds_bytes<16, K, 512>(1 << lg) > 160 * 1024)
+        return kSplitBlock;
     if (c->tune.split_block != 0) return c->tune.split_block;
     return lg >= 7 ? 512 : kSplitBlock;
+}
+// the WG strategy's scatter shape (the chained strategy keeps the base one)
+template <class K>
+int wg_ipt(edpp_ctx* c, int lg) {
+    return wide_u64<K>(c, lg) ? 8 : wg_ipt_base<K>(c, lg);
+}
+template <class K>
+int wg_block(edpp_ctx* c, int lg) {
+    return wide_u64<K>(c, lg) ? 512 : wg_block_base<K>(c, lg);
 }
 
 // histogram entries one pass needs
@@ -803,6 +822,9 @@ int run_pass(edpp_ctx* c, int strategy, const SplitCall<K>& a, int32_t* his) {
                     return bal ? run_wg<K, 8, true, kSplitBlock, kReorderNone>(c, a, his)
                                : run_wg<K, 8, false, kSplitBlock, kReorderNone>(c, a, his);
                 }
+                if constexpr (sizeof(K) == 8) {
+                    if (wide_u64<K>(c, a.lg)) return run_wg<K, 8, false, 512>(c, a, his);  // lg >= 7: no ballots
+                }
                 if (wg_ipt<K>(c, a.lg) == 16) {
                     if constexpr (sizeof(K) == 4) {
                         if (wg_block<K>(c, a.lg) == 512)
@@ -873,7 +895,7 @@ template <class K>
 int launch_chain(edpp_ctx* c, int64_t n, int digits, const ChainStep<K>* steps, int32_t* d_start, uint32_t* small) {
     int maxlg = 0;
     for (int p = 0; p < digits; ++p) maxlg = std::max(maxlg, steps[p].lg);
-    const int ipt = wg_ipt<K>(c, maxlg), block = wg_block<K>(c, maxlg);
+    const int ipt = wg_ipt_base<K>(c, maxlg), block = wg_block_base<K>(c, maxlg);
     const int64_t S = (int64_t)block * ipt;
     ChainRanges cr;
     cr.tiles = ceil_div(n, S);
