@@ -607,13 +607,16 @@ bool reorder_256(edpp_ctx* c) {
     return c->tune.split_reorder == EDPP_REORDER_NONE || c->tune.split_reorder == EDPP_REORDER_FIXED;
 }
 
-// 8-byte keys from 128 buckets at the large-input shape: 512-lane scatter
-// workgroups with 8 keys per lane (the 4096-pair sub-tile of 256 lanes at 16,
-// with values staged after keys and loaded late)
+// 512-lane scatter workgroups with 8 keys per lane (the 4096-pair sub-tile of
+// 256 lanes at 16, with values staged after keys and loaded late) at the
+// large-input shape: 8-byte keys from 128 buckets (B=256, 2^27 pairs: 0.99
+// against 1.04 ms), 4-byte keys at 64 buckets (a 4096-bucket split's two
+// passes: 2.15 against 2.21 ms; at 32 buckets and below 256 lanes are faster)
 template <class K>
-bool wide_u64(edpp_ctx* c, int lg) {
-    return sizeof(K) == 8 && !reorder_256(c) && c->tune.split_ipt >= 16 && c->tune.split_block != kSplitBlock &&
-           lg >= 7 && wg_lds_bytes<8, K, 512>(1 << lg) <= 160 * 1024;
+bool wide_8(edpp_ctx* c, int lg) {
+    if (reorder_256(c) || c->tune.split_ipt < 16 || c->tune.split_block != 0) return false;  // default shape only
+    if (wg_lds_bytes<8, K, 512>(1 << lg) > 160 * 1024) return false;
+    return sizeof(K) == 8 ? lg >= 7 : lg == 6;
 }
 
 template <class K>
@@ -640,11 +643,11 @@ int wg_block_base(edpp_ctx* c, int lg) {
 // the WG strategy's scatter shape (the chained strategy keeps the base one)
 template <class K>
 int wg_ipt(edpp_ctx* c, int lg) {
-    return wide_u64<K>(c, lg) ? 8 : wg_ipt_base<K>(c, lg);
+    return wide_8<K>(c, lg) ? 8 : wg_ipt_base<K>(c, lg);
 }
 template <class K>
 int wg_block(edpp_ctx* c, int lg) {
-    return wide_u64<K>(c, lg) ? 512 : wg_block_base<K>(c, lg);
+    return wide_8<K>(c, lg) ? 512 : wg_block_base<K>(c, lg);
 }
 
 // histogram entries one pass needs
@@ -822,9 +825,7 @@ int run_pass(edpp_ctx* c, int strategy, const SplitCall<K>& a, int32_t* his) {
                     return bal ? run_wg<K, 8, true, kSplitBlock, kReorderNone>(c, a, his)
                                : run_wg<K, 8, false, kSplitBlock, kReorderNone>(c, a, his);
                 }
-                if constexpr (sizeof(K) == 8) {
-                    if (wide_u64<K>(c, a.lg)) return run_wg<K, 8, false, 512>(c, a, his);  // lg >= 7: no ballots
-                }
+                if (wide_8<K>(c, a.lg)) return run_wg<K, 8, false, 512>(c, a, his);  // lg >= 6: no ballots
                 if (wg_ipt<K>(c, a.lg) == 16) {
                     if constexpr (sizeof(K) == 4) {
                         if (wg_block<K>(c, a.lg) == 512)

@@ -26,9 +26,10 @@ typedef struct edpp_tuning {
     int32_t split_ipt;        /* keys per lane per split sub-tile: 8 or 16; 0 = default                  */
     int32_t split_tile_mult;  /* split tile >= mult * buckets elements: 16..1024; 0 = default (16)       */
     int32_t scan_block;       /* lanes per look-back workgroup: 256, 512, 1024; 0 = default              */
-    int32_t split_block;      /* lanes per WG-split scatter workgroup: 256, or 512 (values staged after
-                                 keys; 16 keys per lane of 4 bytes, 8 of 8 bytes); 0 = default (512 from
-                                 128 buckets at the large-input shape)                                  */
+    int32_t split_block;      /* lanes per WG-split scatter workgroup: 256, or 512 (4-byte keys, values
+                                 staged after keys, 16 keys per lane); 0 = default: at the large-input
+                                 shape 512 lanes from 128 buckets, and 512 lanes with 8 keys per lane
+                                 for 8-byte keys from 128 buckets and 4-byte keys at 64               */
     int32_t split_reorder;    /* how the WG split writes a tile (the reference's ReorderType,
                                  opencl/types.h:43-45): EDPP_REORDER_VARIED (the default), _NONE or
                                  _FIXED; 0 = default                                                  */
