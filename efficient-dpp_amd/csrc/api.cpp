@@ -173,13 +173,13 @@ int default_passes(edpp_ctx* c, int64_t n, int elem_bytes, bool scatter) {
 }
 // The look-back scan is the fastest scheme at every size for both widths
 // (2^14..2^28 measured); reduce-then-scan ties it only near 2^21-2^22.
-// Tiles of 1024 lanes x 4 vectors below 2^21 elements, 1024 x 8 above.
-// From 2^25 elements the persistent look-back wins (2^26 int32: 102 against
-// 111 us; 2^28: 375 against 421 us, a copy takes 370 us); below it one
-// workgroup per tile (2^24: 30.0 against 34.6 us).
+// Above 2^25 elements the persistent look-back wins (3 * 2^24 int32: 78
+// against 86 us; 2^26: 101 against 110 us; 2^28: 375 against 421 us, a copy
+// takes 370 us); up to 2^25 one workgroup per tile (2^25: 55 against 57 us;
+// 2^24: 30.0 against 34.6 us).  The tile shape is default_scan_shape's.
 int default_scan_scheme(edpp_ctx* c, int64_t n, int elem_bytes) {
     (void)c; (void)elem_bytes;
-    return n >= (1ll << 25) ? EDPP_SCAN_PERSIST : EDPP_SCAN_LOOKBACK;
+    return n > (1ll << 25) ? EDPP_SCAN_PERSIST : EDPP_SCAN_LOOKBACK;
 }
 // Look-back tile of the scan from a sweep of 2^22..2^28 int32 (1024 lanes;
 // larger tiles mean fewer look-backs, which the single workgroup per CU of

@@ -171,14 +171,15 @@ def test_model_gain_and_model_driven_calls(torch_gpu, ctx, edpp, orc):
 def test_default_candidate_is_the_builtin_schedule(torch_gpu, ctx, edpp):
     torch = torch_gpu
     tn = importlib.import_module("efficient-dpp_amd.tuner")
-    ws = tn.Workspace(edpp, ctx, torch, 1 << 25)
+    ws = tn.Workspace(edpp, ctx, torch, 1 << 26)
     names = [c[1] for c in edpp.candidates()]
     want = {
         (0, 1 << 16, 4, 256): "scan look-back 1024x4",
         (0, 1 << 22, 8, 256): "scan look-back 1024x8",
         (0, 1 << 24, 4, 256): "scan look-back 1024x16",
         (0, 1 << 24, 8, 256): "scan look-back 1024x16",  # runs 1024x8 for 64-bit values
-        (0, 1 << 25, 4, 256): "scan persistent look-back 1024x8",
+        (0, 1 << 25, 4, 256): "scan look-back 1024x16",
+        (0, 1 << 26, 4, 256): "scan persistent look-back 1024x8",
         (1, 1 << 20, 4, 256): "gather/scatter 1 pass, unroll 4",
         (2, 1 << 25, 4, 256): "gather/scatter 2 passes",
         (3, 1 << 20, 4, 256): "split WG 8/lane, tile 16B",
