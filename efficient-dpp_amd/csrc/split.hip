@@ -631,7 +631,8 @@ int wg_ipt_base(edpp_ctx* c, int lg) {
 // after keys) from 128 buckets on: a bucket's run per sub-tile doubles, so
 // fewer stores are partial cache lines; only for 4-byte keys at 16 per lane.
 // B=128, 2^28 pairs: 1.12 against 1.22 ms uniform, 1.13 against 1.16 ms
-// Zipf.  Up to 64 buckets the 256-lane workgroup is as fast or faster.
+// Zipf.  Up to 64 buckets 256 lanes at 16 keys are as fast as 512 at 16 or
+// faster (at 64, wide_8's 512 lanes at 8 keys are faster still).
 template <class K>
 int wg_block_base(edpp_ctx* c, int lg) {
     if (reorder_256(c)) return kSplitBlock;
