@@ -75,61 +75,105 @@ __device__ __forceinline__ uint32_t join_lookup(const uint64_t* table, uint32_t 
     }
 }
 
-// One workgroup per partition pair b: R rows [rs[b], rs[b+1]) and S rows
-// [ss[b], ss[b+1]) of the partitioned mixed keys.
-// A table fill holds at most kJoinChunk distinct keys in kJoinSlots slots, so
-// every probe sequence meets an empty slot and ends.
+// Rows of partition pair b: R rows [rs[b], rs[b+1]) and S rows [ss[b],
+// ss[b+1]) of the partitioned mixed keys, the smaller side built into the
+// table.  The offsets are clamped to the relations: a split whose device
+// wait timed out (reported as EDPP_ETIMEOUT) leaves offsets that must not
+// steer reads.
+struct JoinPair {
+    const uint32_t* bkeys;
+    const uint32_t* pkeys;
+    int64_t b0, b1, p0, p1;
+    // both sides fit the registers of one table fill (the balanced case)
+    __device__ __forceinline__ bool held() const {
+        return b1 > b0 && p1 > p0 && b1 - b0 <= kJoinPre * kJoinBlock && p1 - p0 <= kJoinPre * kJoinBlock;
+    }
+};
+__device__ __forceinline__ JoinPair join_pair(int b, const uint32_t* rk, int64_t rn, const int32_t* rs,
+                                              const uint32_t* sk, int64_t sn, const int32_t* ss, int B) {
+    auto clamp = [](int64_t x, int64_t lo, int64_t hi) { return x < lo ? lo : (x > hi ? hi : x); };
+    const int64_t r0 = clamp(rs[b], 0, rn), r1 = clamp(b + 1 < B ? rs[b + 1] : rn, r0, rn);
+    const int64_t s0 = clamp(ss[b], 0, sn), s1 = clamp(b + 1 < B ? ss[b + 1] : sn, s0, sn);
+    const bool build_r = (r1 - r0) <= (s1 - s0);
+    JoinPair jp;
+    jp.bkeys = build_r ? rk : sk;
+    jp.pkeys = build_r ? sk : rk;
+    jp.b0 = build_r ? r0 : s0;
+    jp.b1 = build_r ? r1 : s1;
+    jp.p0 = build_r ? s0 : r0;
+    jp.p1 = build_r ? s1 : r1;
+    return jp;
+}
+// lane's keys of a held pair's two sides
+__device__ __forceinline__ void join_load(const JoinPair& jp, uint32_t (&bk)[kJoinPre], uint32_t (&pk)[kJoinPre]) {
+    #pragma unroll
+    for (int q = 0; q < kJoinPre; ++q) {
+        const int64_t i = jp.b0 + threadIdx.x + (int64_t)q * kJoinBlock;
+        bk[q] = i < jp.b1 ? jp.bkeys[i] : 0u;
+    }
+    #pragma unroll
+    for (int q = 0; q < kJoinPre; ++q) {
+        const int64_t i = jp.p0 + threadIdx.x + (int64_t)q * kJoinBlock;
+        pk[q] = i < jp.p1 ? jp.pkeys[i] : 0u;
+    }
+}
+
+// Persistent: workgroup g (one per CU, as its table fills the CU's LDS)
+// takes partition pairs g, g + gridDim.x, ...  While a pair's table is
+// cleared, filled and probed, the next pair's offsets and (when both of its
+// sides fit the registers) keys are loaded, so their global latency is hidden
+// behind the LDS work.  A table fill holds at most kJoinChunk distinct keys in
+// kJoinSlots slots, so every probe sequence meets an empty slot and ends.
 __global__ __launch_bounds__(kJoinBlock) void k_join_probe(const uint32_t* __restrict__ rk, int64_t rn,
                                                            const int32_t* __restrict__ rs,
                                                            const uint32_t* __restrict__ sk, int64_t sn,
                                                            const int32_t* __restrict__ ss, int B, int pbits,
                                                            unsigned long long* d_count) {
     extern __shared__ uint64_t table[];  // kJoinSlots
-    const int b = blockIdx.x;
-    // offsets clamped to the relations: a split whose device wait timed out
-    // (reported as EDPP_ETIMEOUT) leaves offsets that must not steer reads
-    auto clamp = [](int64_t x, int64_t lo, int64_t hi) { return x < lo ? lo : (x > hi ? hi : x); };
-    const int64_t r0 = clamp(rs[b], 0, rn), r1 = clamp(b + 1 < B ? rs[b + 1] : rn, r0, rn);
-    const int64_t s0 = clamp(ss[b], 0, sn), s1 = clamp(b + 1 < B ? ss[b + 1] : sn, s0, sn);
-    const bool build_r = (r1 - r0) <= (s1 - s0);
-    const uint32_t* bkeys = build_r ? rk : sk;
-    const uint32_t* pkeys = build_r ? sk : rk;
-    const int64_t b0 = build_r ? r0 : s0, b1 = build_r ? r1 : s1;
-    const int64_t p0 = build_r ? s0 : r0, p1 = build_r ? s1 : r1;
     unsigned long long local = 0;
-    if (b1 > b0 && p1 > p0 && b1 - b0 <= kJoinPre * kJoinBlock && p1 - p0 <= kJoinPre * kJoinBlock) {
-        // one table fill: both sides' keys are loaded into registers first, so
-        // their global latency overlaps the table clearing (the workgroup
-        // holds the CU's LDS alone, and no other workgroup hides it)
-        uint32_t bk[kJoinPre], pk[kJoinPre];
-        #pragma unroll
-        for (int q = 0; q < kJoinPre; ++q) {
-            const int64_t i = b0 + threadIdx.x + (int64_t)q * kJoinBlock;
-            bk[q] = i < b1 ? bkeys[i] : 0u;
-        }
-        #pragma unroll
-        for (int q = 0; q < kJoinPre; ++q) {
-            const int64_t i = p0 + threadIdx.x + (int64_t)q * kJoinBlock;
-            pk[q] = i < p1 ? pkeys[i] : 0u;
-        }
-        for (int i = threadIdx.x; i < kJoinSlots; i += kJoinBlock) table[i] = 0ull;
-        __syncthreads();
-        #pragma unroll
-        for (int q = 0; q < kJoinPre; ++q)
-            if (b0 + threadIdx.x + (int64_t)q * kJoinBlock < b1) join_insert(table, bk[q], pbits);
-        __syncthreads();
-        #pragma unroll
-        for (int q = 0; q < kJoinPre; ++q)
-            if (p0 + threadIdx.x + (int64_t)q * kJoinBlock < p1) local += join_lookup(table, pk[q], pbits);
-    } else if (b1 > b0 && p1 > p0) {
-        for (int64_t c0 = b0; c0 < b1; c0 += kJoinChunk) {
-            const int64_t c1 = std::min<int64_t>(c0 + kJoinChunk, b1);
+    int b = blockIdx.x;
+    JoinPair cur = join_pair(b, rk, rn, rs, sk, sn, ss, B);
+    uint32_t bk[kJoinPre], pk[kJoinPre];
+    if (cur.held()) join_load(cur, bk, pk);
+    for (; b < B; b += gridDim.x) {
+        const int nb = b + (int)gridDim.x;
+        JoinPair nxt = cur;
+        if (nb < B) nxt = join_pair(nb, rk, rn, rs, sk, sn, ss, B);
+        uint32_t nbk[kJoinPre], npk[kJoinPre];
+        __syncthreads();  // the previous pair's lookups are done before the table is cleared
+        if (cur.held()) {
             for (int i = threadIdx.x; i < kJoinSlots; i += kJoinBlock) table[i] = 0ull;
+            if (nb < B && nxt.held()) join_load(nxt, nbk, npk);
             __syncthreads();
-            for (int64_t i = c0 + threadIdx.x; i < c1; i += kJoinBlock) join_insert(table, bkeys[i], pbits);
+            #pragma unroll
+            for (int q = 0; q < kJoinPre; ++q)
+                if (cur.b0 + threadIdx.x + (int64_t)q * kJoinBlock < cur.b1) join_insert(table, bk[q], pbits);
             __syncthreads();
-            for (int64_t i = p0 + threadIdx.x; i < p1; i += kJoinBlock) local += join_lookup(table, pkeys[i], pbits);
-            __syncthreads();
+            #pragma unroll
+            for (int q = 0; q < kJoinPre; ++q)
+                if (cur.p0 + threadIdx.x + (int64_t)q * kJoinBlock < cur.p1)
+                    local += join_lookup(table, pk[q], pbits);
+        } else {
+            if (nb < B && nxt.held()) join_load(nxt, nbk, npk);
+            if (cur.b1 > cur.b0 && cur.p1 > cur.p0) {
+                for (int64_t c0 = cur.b0; c0 < cur.b1; c0 += kJoinChunk) {
+                    const int64_t c1 = std::min<int64_t>(c0 + kJoinChunk, cur.b1);
+                    for (int i = threadIdx.x; i < kJoinSlots; i += kJoinBlock) table[i] = 0ull;
+                    __syncthreads();
+                    for (int64_t i = c0 + threadIdx.x; i < c1; i += kJoinBlock)
+                        join_insert(table, cur.bkeys[i], pbits);
+                    __syncthreads();
+                    for (int64_t i = cur.p0 + threadIdx.x; i < cur.p1; i += kJoinBlock)
+                        local += join_lookup(table, cur.pkeys[i], pbits);
+                    __syncthreads();
+                }
+            }
+        }
+        cur = nxt;
+        #pragma unroll
+        for (int q = 0; q < kJoinPre; ++q) {
+            bk[q] = nbk[q];
+            pk[q] = npk[q];
         }
     }
     // workgroup sum: wave sums, then one lane per wave into the table's first slots
@@ -175,8 +219,10 @@ int launch_join_count(edpp_ctx* c, const uint32_t* rk, int64_t rn, const uint32_
     if (rc) return rc;
     const size_t lds = (size_t)kJoinSlots * 8;
     EDPP_HIP(c, hipFuncSetAttribute((const void*)k_join_probe, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_join_probe, dim3((unsigned)B), dim3(kJoinBlock), lds, c->stream, rpart, rn, rst, spart, sn,
-                       sst, B, pbits, d_count);
+    // one persistent workgroup per CU (its table holds the CU's LDS)
+    const unsigned grid = (unsigned)std::max(1, std::min(B, std::max(c->cus, 1)));
+    hipLaunchKernelGGL(k_join_probe, dim3(grid), dim3(kJoinBlock), lds, c->stream, rpart, rn, rst, spart, sn, sst, B,
+                       pbits, d_count);
     EDPP_HIP(c, hipGetLastError());
     return EDPP_OK;
 }
