@@ -430,8 +430,10 @@ def test_split_wg_shapes(torch_gpu, ctx, orc, ipt, mult, block, B):
 
 
 @pytest.mark.parametrize("strategy", [1, 2, 4])
-@pytest.mark.parametrize("B", [2, 256, 4096])
+@pytest.mark.parametrize("B", [2, 128, 256, 4096])
 def test_split_i64(torch_gpu, ctx, orc, strategy, B):
+    # from 128 buckets the WG strategy runs 8-byte keys on 512 lanes at 8 keys
+    # per lane (values staged after the keys)
     torch = torch_gpu
     n = 300_007
     keys, vals = orc.gen_split_keys(n, B, 1, 5 + B, wide=True)
