@@ -225,6 +225,26 @@ __device__ void split_hist_tile(const DJob& J, int64_t t, uint32_t* lds) {
     }
 }
 
+// place_and_store hook of a split job of one tile: its output offsets are the
+// tile's own exclusive bucket prefix (off[]), so the job needs no histogram
+// or histogram-scan tile.  Thread b sets run[b] (and the job's bucket start)
+// and then reads it itself.
+struct SelfOffsets {
+    static constexpr bool kSync = false;
+    const uint32_t* off;
+    uint32_t* run;
+    int32_t* aux;
+    int B;
+    __device__ __forceinline__ void publish() const {}
+    __device__ __forceinline__ void keys_placed() const {}
+    __device__ __forceinline__ void resolve() const {
+        for (int b = threadIdx.x; b < B; b += blockDim.x) {
+            run[b] = off[b];
+            if (aux) aux[b] = (int32_t)off[b];
+        }
+    }
+};
+
 template <class K, int IPT, int BLOCK>
 __device__ void split_scatter_tile(const DJob& J, int64_t t, uint32_t* lds, uint32_t spin_limit, int32_t* err) {
     constexpr int W = BLOCK / kWave;
@@ -240,10 +260,13 @@ __device__ void split_scatter_tile(const DJob& J, int64_t t, uint32_t* lds, uint
     int32_t* vbuf = reinterpret_cast<int32_t*>(kbuf + S);
     const int w = threadIdx.x / kWave;
     const bool kv = J.vin != nullptr;
-    for (int b = threadIdx.x; b < B; b += BLOCK) {
-        const uint32_t r = wait_tagged(J.his + (int64_t)b * J.tiles + t, kTagScanned, spin_limit, err);
-        run[b] = r;
-        if (t == 0 && J.aux) static_cast<int32_t*>(J.aux)[b] = (int32_t)r;
+    const bool single = J.tiles == 1;
+    if (!single) {
+        for (int b = threadIdx.x; b < B; b += BLOCK) {
+            const uint32_t r = wait_tagged(J.his + (int64_t)b * J.tiles + t, kTagScanned, spin_limit, err);
+            run[b] = r;
+            if (t == 0 && J.aux) static_cast<int32_t*>(J.aux)[b] = (int32_t)r;
+        }
     }
     for (int b = threadIdx.x; b < W * B; b += BLOCK) {
         whist[b] = 0;
@@ -262,12 +285,23 @@ __device__ void split_scatter_tile(const DJob& J, int64_t t, uint32_t* lds, uint
                                     key, val);
     __syncthreads();
     const int cnt = (int)(tend - s);
-    if (kv)
+    if (single) {
+        const SelfOffsets hook{off, run, static_cast<int32_t*>(J.aux), B};
+        if (kv)
+            place_and_store<K, IPT, true, BLOCK, false, SelfOffsets>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk,
+                                                                     key, val, B, 0, cnt, static_cast<K*>(J.out),
+                                                                     J.vout, J.n, 0u, hook);
+        else
+            place_and_store<K, IPT, false, BLOCK, false, SelfOffsets>(whist, run, sub, off, wtmp, kbuf, vbuf, bk,
+                                                                      rk, key, val, B, 0, cnt, static_cast<K*>(J.out),
+                                                                      nullptr, J.n, 0u, hook);
+    } else if (kv) {
         place_and_store<K, IPT, true, BLOCK>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk, key, val, B, 0, cnt,
                                              static_cast<K*>(J.out), J.vout, J.n);
-    else
+    } else {
         place_and_store<K, IPT, false, BLOCK>(whist, run, sub, off, wtmp, kbuf, vbuf, bk, rk, key, val, B, 0, cnt,
                                               static_cast<K*>(J.out), nullptr, J.n);
+    }
 }
 
 // ---------------------------------------------------------------- gather tiles
@@ -540,7 +574,8 @@ static int plan(edpp_batch* b) {
         soff[j] = words;
         if (q.kind == EDPP_JOB_SCAN) {
             words += (d.wide ? 4 : 2) * d.tiles;  // one or two u64 status words per tile
-        } else if (q.kind == EDPP_JOB_SPLIT) {
+        } else if (q.kind == EDPP_JOB_SPLIT && d.tiles > 1) {
+            // (a job of one tile takes its offsets from its own counts)
             const int64_t hlen = ((int64_t)1 << d.lg) * d.tiles;
             d.hscan_tiles = ceil_div(hlen, hscan_tile_elems(sh));
             words += 2 * d.hscan_tiles + 2 * hlen;  // u64 status, then u64 histogram words
@@ -571,7 +606,7 @@ static int plan(edpp_batch* b) {
     // the ticket table, queue after queue
     int64_t all_tiles = 0;
     for (int32_t j = 0; j < njobs; ++j)  // a split: histogram, histogram-scan and scatter tiles
-        all_tiles += dj[j].tiles + (dj[j].kind == EDPP_JOB_SPLIT ? dj[j].tiles + dj[j].hscan_tiles : 0);
+        all_tiles += dj[j].tiles + (dj[j].kind == EDPP_JOB_SPLIT && dj[j].tiles > 1 ? dj[j].tiles + dj[j].hscan_tiles : 0);
     if (all_tiles > kMaxBatchTiles)
         return fail(c, EDPP_EUNSUP, "batch of %lld tiles is too large (at most %lld)", (long long)all_tiles,
                     (long long)kMaxBatchTiles);
@@ -589,9 +624,12 @@ static int plan(edpp_batch* b) {
         tk = 0;
         qt.tix_first[q] = (int64_t)tix.size();
         for (int32_t j = 0; j < njobs; ++j)
-            if (qof[j] == q && dj[j].kind == EDPP_JOB_SPLIT) { add(j, kPhaseHist, dj[j].tiles); hist += dj[j].tiles; }
+            if (qof[j] == q && dj[j].kind == EDPP_JOB_SPLIT && dj[j].tiles > 1) {
+                add(j, kPhaseHist, dj[j].tiles);
+                hist += dj[j].tiles;
+            }
         for (int32_t j = 0; j < njobs; ++j)
-            if (qof[j] == q && dj[j].kind == EDPP_JOB_SPLIT) {
+            if (qof[j] == q && dj[j].kind == EDPP_JOB_SPLIT && dj[j].tiles > 1) {
                 add(j, kPhaseHistScan, dj[j].hscan_tiles);
                 hscan += dj[j].hscan_tiles;
             }
