@@ -1,12 +1,14 @@
 // batch.hip — persistent executor for many scans, splits and gathers of
 // uneven length in one launch, fed by a device ticket queue.
 //
-// Queue order (tickets):  [split histogram tiles of every split job]
+// Queue order (tickets):  [split histogram tiles of every split job of
+//                          more than one tile]
 //                         [tiles of the in-place scans of those histograms]
 //                         [scan tiles and gather tiles, in job order]
 //                         [split scatter tiles]
 // (the PHASES order; the JOBS order puts each split's scatter tiles at its
-// job's place among the scan and gather tiles).  Every dependency points to
+// job's place among the scan and gather tiles).  A split job of one tile has
+// only its scatter tile, which takes its offsets from its own counts.  Every dependency points to
 // smaller tickets: a histogram scan waits for its job's histogram tiles, a
 // scatter tile for its job's histogram scan, a scan tile (and a
 // histogram-scan tile) for its predecessors in the look-back.  Tickets are
