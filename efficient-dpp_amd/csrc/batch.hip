@@ -8,12 +8,12 @@
 //                         [split scatter tiles]
 // (the PHASES order; the JOBS order puts each split's scatter tiles at its
 // job's place among the scan and gather tiles).  A split job of one tile has
-// only its scatter tile, which takes its offsets from its own counts.  Every dependency points to
-// smaller tickets: a histogram scan waits for its job's histogram tiles, a
-// scatter tile for its job's histogram scan, a scan tile (and a
-// histogram-scan tile) for its predecessors in the look-back.  Tickets are
-// handed out in order, so whatever a workgroup waits for is held by a
-// workgroup that is already running; every wait ends.  In the XCD order the
+// only its scatter tile, which takes its offsets from its own counts.
+// Every dependency points to smaller tickets: a histogram scan waits for its
+// job's histogram tiles, a scatter tile for its job's histogram scan, a scan
+// tile (and a histogram-scan tile) for its predecessors in the look-back.
+// Tickets are handed out in order, so whatever a workgroup waits for is held
+// by a workgroup that is already running; every wait ends.  In the XCD order the
 // jobs are dealt to one queue per XCD (each queue in the JOBS order over its
 // own jobs, with its own ticket counter); workgroup g starts on queue
 // g % xcds, the queue of the XCD it runs on, so a job's tiles share one L2,

@@ -5,7 +5,8 @@
  * each (opencl/primitives/*.cpp: every launch followed by clFinish).  A batch
  * is planned once: every job is cut into tiles, the tiles form one device
  * work queue (split histograms first, then the scans of those histograms,
- * then scan, gather and split-scatter tiles), and one persistent kernel
+ * then scan, gather and split-scatter tiles; a split job of one tile has
+ * only its scatter tile), and one persistent kernel
  * drains it.  A tile only ever waits for tiles with smaller queue tickets,
  * which belong to workgroups that are already running, so the launch always
  * completes.  Results are bit-exact per job with the single-call API.
@@ -83,7 +84,8 @@ int edpp_batch_get_shape(edpp_batch* b, int32_t* shape, int32_t* wgs_per_cu, int
  * [order][shape][occupancy].  Runs the batch repeatedly, so jobs writing
  * their own input are rejected (EDPP_EINVAL). */
 int edpp_batch_autotune(edpp_batch* b, int32_t reps, double* times);
-/* Queue statistics: total tiles, split histogram tiles, histogram-scan tiles. */
+/* Queue statistics: total tiles, split histogram tiles, histogram-scan tiles
+ * (split jobs of one tile have neither). */
 int edpp_batch_info(edpp_batch* b, int64_t* tiles, int64_t* hist_tiles, int64_t* hscan_tiles);
 
 /* The baseline the batch replaces: one ordinary library call per job. */
