@@ -675,11 +675,15 @@ int run_wg(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
     g.flip = a.flip;
     const int64_t hlen = (int64_t)B * g.tiles;
     // histogram: register counters up to 16 buckets; above, LDS atomics with
-    // four histogram sub-tiles per workgroup where they make whole tiles
+    // four histogram sub-tiles per workgroup where they make whole tiles and
+    // still leave 32 workgroups per CU, two below that (more workgroups then
+    // share the CUs)
     const int regw = a.lg <= kRegCountBits ? reg_count_words(a.lg) : 0;
     const int64_t spt = g.tile / SH;
-    const bool multi = regw == 0 && spt <= kHistTilesPerWG && (spt & (spt - 1)) == 0;
-    const int tpw = multi ? kHistTilesPerWG / (int)spt : 1;
+    const int hsub =
+        ceil_div(g.tiles * spt, (int64_t)kHistTilesPerWG) >= 32 * (int64_t)c->cus ? kHistTilesPerWG : 2;
+    const bool multi = regw == 0 && spt <= hsub && (spt & (spt - 1)) == 0;
+    const int tpw = multi ? hsub / (int)spt : 1;
     const size_t lh = (size_t)tpw * kSplitWaves * B * 4;
     const size_t ls = RO == kReorderFixed ? fixed_lds_bytes<K>(B, S)
                       : RO == kReorderNone ? (size_t)kbuf_word_offset(B) * 4
@@ -695,7 +699,10 @@ int run_wg(edpp_ctx* c, const SplitCall<K>& a, int32_t* his) {
         case 2: hipLaunchKernelGGL((k_split_hist<K, IPT, 1, 2>), dim3(hgrid), dim3(kSplitBlock), lh, c->stream, a.kin, g, 1, his); break;
         case 4: hipLaunchKernelGGL((k_split_hist<K, IPT, 1, 4>), dim3(hgrid), dim3(kSplitBlock), lh, c->stream, a.kin, g, 1, his); break;
         default:
-            if (multi)
+            if (multi && hsub == 2)
+                hipLaunchKernelGGL((k_split_hist<K, IPT, 2, 0>), dim3(hgrid), dim3(kSplitBlock), lh,
+                                   c->stream, a.kin, g, ispt, his);
+            else if (multi)
                 hipLaunchKernelGGL((k_split_hist<K, IPT, kHistTilesPerWG, 0>), dim3(hgrid), dim3(kSplitBlock), lh,
                                    c->stream, a.kin, g, ispt, his);
             else
