@@ -543,21 +543,67 @@ __global__ void k_split_single_scatter(const K* kin, K* kout, const int32_t* vin
 }
 
 // d_start[b] = first position of bucket b in the split output kout, which is
-// ordered by bucket: a binary search per bucket (a bucket absent from the
-// output starts where the next present one does, or at n)
-template <class K>
+// ordered by bucket (a bucket absent from the output starts where the next
+// present one does, or at n).  A group of G lanes searches for one bucket,
+// kWave/G buckets per wave, (G+1)-way: every round the group's lanes test G
+// evenly spaced positions of [lo, hi) at once, and one ballot keeps the gap
+// between the last position below the bucket and the first one at or above
+// it.  Invariant: every key before lo lies in a smaller bucket, every key
+// from hi on in bucket b or a larger one.
+template <class K, int G>
 __global__ void k_split_start_search(const K* __restrict__ kout, int64_t n, int lg, int32_t* __restrict__ d_start) {
+    constexpr int GPW = kWave / G;  // buckets per wave
+    constexpr uint64_t kGroupMask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
     const int B = 1 << lg;
     const uint32_t mask = (uint32_t)B - 1u;
-    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < B; b += gridDim.x * blockDim.x) {
+    const int lane = lane_id();
+    const int gl = lane % G, grp = lane / G;
+    const int64_t waves = (int64_t)gridDim.x * blockDim.x / kWave;
+    for (int64_t b0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave * GPW; b0 < B; b0 += waves * GPW) {
+        // a group past the last bucket (B < GPW) searches for b >= B: every
+        // key lies below it, every position it loads lies in [0, n), and its
+        // result is not stored
+        const uint32_t b = (uint32_t)b0 + (uint32_t)grp;
         int64_t lo = 0, hi = n;
-        while (lo < hi) {
-            const int64_t mid = lo + (hi - lo) / 2;
-            if (bucket_of(kout[mid], 0, mask, 0u) < (uint32_t)b) lo = mid + 1;
-            else hi = mid;
+        while (ballot(hi - lo > G)) {
+            const int64_t len = hi - lo;
+            const bool act = len > G;
+            bool below = false;
+            if (act) {
+                // positions lo + (l+1)*len/(G+1), l = 0..G-1: strictly
+                // increasing (len > G) and all inside [lo, hi)
+                const int64_t p = lo + ((int64_t)(gl + 1) * len) / (G + 1);
+                below = bucket_of(kout[p], 0, mask, 0u) < b;
+            }
+            // positions below b: the group's lanes 0..c-1
+            const int c = __popcll((ballot(below) >> (grp * G)) & kGroupMask);
+            if (act) {
+                const int64_t nlo = c > 0 ? lo + ((int64_t)c * len) / (G + 1) + 1 : lo;
+                const int64_t nhi = c < G ? lo + ((int64_t)(c + 1) * len) / (G + 1) : hi;
+                lo = nlo;
+                hi = nhi;
+            }
         }
-        d_start[b] = (int32_t)lo;
+        // at most G positions left: one test each
+        const int64_t e = lo + gl;
+        const bool below = e < hi && bucket_of(kout[e], 0, mask, 0u) < b;
+        const int c = __popcll((ballot(below) >> (grp * G)) & kGroupMask);
+        if (gl == 0 && b < (uint32_t)B) d_start[b] = (int32_t)(lo + c);
     }
+}
+
+// launches the bucket-start search with 16 lanes per bucket (17-way rounds).
+// Lanes per bucket trade rounds of dependent loads against lines fetched:
+// over 2^24 keys 64 lanes take 3 rounds and a final test but fetch about 64
+// lines per bucket in their last round, 16 lanes take 5 rounds and fetch at
+// most 16 lines per round; a join of 2^24 x 2^24 keys (4096 partitions) measured 450 us with
+// 16 or 8 lanes, 454 with 4 and 457 with 64.
+template <class K>
+void launch_start_search(edpp_ctx* c, const K* kout, int64_t n, int lg, int32_t* d_start) {
+    constexpr int G = 16;
+    const int64_t B = (int64_t)1 << lg;
+    const unsigned grid = (unsigned)ceil_div(B, (int64_t)(256 / kWave) * (kWave / G));
+    hipLaunchKernelGGL((k_split_start_search<K, G>), dim3(grid), dim3(256), 0, c->stream, kout, n, lg, d_start);
 }
 
 __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
@@ -1016,8 +1062,7 @@ int launch_split(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t
         rc = launch_chain<K>(c, n, 2, st, nullptr, small);
         if (rc) return rc;
         if (d_start) {
-            hipLaunchKernelGGL((k_split_start_search<K>), dim3((unsigned)ceil_div(buckets, 256)), dim3(256), 0,
-                               c->stream, kout, n, lg, d_start);
+            launch_start_search<K>(c, kout, n, lg, d_start);
             EDPP_HIP(c, hipGetLastError());
         }
         return EDPP_OK;
@@ -1050,8 +1095,7 @@ int launch_split(edpp_ctx* c, const K* kin, K* kout, const int32_t* vin, int32_t
     rc = run_pass<K2>(c, strategy, a2, his);
     if (rc) return rc;
     if (d_start) {
-        hipLaunchKernelGGL((k_split_start_search<K2>), dim3((unsigned)ceil_div(buckets, 256)), dim3(256), 0, c->stream,
-                           kout2, n, lg, d_start);
+        launch_start_search<K2>(c, kout2, n, lg, d_start);
         EDPP_HIP(c, hipGetLastError());
     }
     return EDPP_OK;
