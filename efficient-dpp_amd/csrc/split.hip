@@ -596,8 +596,8 @@ __global__ void k_split_start_search(const K* __restrict__ kout, int64_t n, int 
 // Lanes per bucket trade rounds of dependent loads against lines fetched:
 // over 2^24 keys 64 lanes take 3 rounds and a final test but fetch about 64
 // lines per bucket in their last round, 16 lanes take 5 rounds and fetch at
-// most 16 lines per round; a join of 2^24 x 2^24 keys (4096 partitions) measured 450 us with
-// 16 or 8 lanes, 454 with 4 and 457 with 64.
+// most 16 lines per round.  A join of 2^24 x 2^24 keys (4096 partitions)
+// measured 450 us with 16 or 8 lanes, 454 with 4 and 457 with 64.
 template <class K>
 void launch_start_search(edpp_ctx* c, const K* kout, int64_t n, int lg, int32_t* d_start) {
     constexpr int G = 16;
