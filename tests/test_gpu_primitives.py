@@ -348,6 +348,31 @@ def test_split_i32(torch_gpu, ctx, orc, strategy, B, n):
         assert np.array_equal(host(kout2), ko)
 
 
+@pytest.mark.parametrize("strategy", [0, 1, 4])  # AUTO, WG, CHAINED
+def test_split_starts_with_absent_buckets(torch_gpu, ctx, orc, strategy):
+    # 4096 buckets take two passes, whose bucket starts come from a search of
+    # the output: most buckets are absent here (the first and the last among
+    # them), and one bucket holds half of the keys
+    torch = torch_gpu
+    B = 4096
+    n = (1 << 22) + 3
+    rng = np.random.default_rng(4096 + strategy)
+    present = np.sort(rng.choice(np.arange(1, B - 1), 37, replace=False))
+    bk = present[rng.integers(0, len(present), n)]
+    bk[: n // 2] = present[5]
+    keys = ((rng.integers(0, 1 << 19, n, dtype=np.int64) << 12) | bk).astype(np.int32)
+    vals = np.arange(n, dtype=np.int32)
+    ko, vo, start = orc.split(keys, B, vals)
+    assert start[0] == 0 and start[-1] == n  # absent first and last buckets
+    kout = empty(torch, n, torch.int32)
+    vout = empty(torch, n, torch.int32)
+    st = torch.full((B,), -1, dtype=torch.int32, device="cuda")
+    ctx.split(dev(torch, keys), kout, B, dev(torch, vals), vout, st, n, strategy)
+    assert np.array_equal(host(kout), ko), strategy
+    assert np.array_equal(host(vout), vo), strategy
+    assert np.array_equal(host(st), start), strategy
+
+
 @pytest.mark.parametrize("reorder", [1, 2, 3])  # REORDER_NONE, _FIXED, _VARIED
 @pytest.mark.parametrize("ipt,mult", [(8, 16), (16, 16), (16, 64), (8, 256)])
 @pytest.mark.parametrize("B", [1, 2, 16, 32, 256, 4096])
