@@ -44,24 +44,21 @@ __device__ __forceinline__ uint32_t join_slot(uint32_t hk, int pbits) {
     return (pbits >= 32 ? 0u : (hk >> pbits)) & (uint32_t)(kJoinSlots - 1);
 }
 
+// A claim is tried first: on an empty slot (most of them, at a load factor
+// of at most 1/2) it is the only LDS operation; a failed claim returns the
+// slot's word, whose key never changes once set.
 __device__ __forceinline__ void join_insert(uint64_t* table, uint32_t hk, int pbits) {
     uint32_t h = join_slot(hk, pbits);
-    uint64_t w = __hip_atomic_load(table + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     while (true) {
-        if ((w >> 32) == 0) {  // empty: claim it for this key with count 1
-            uint64_t expect = 0;
-            if (__hip_atomic_compare_exchange_strong(table + h, &expect, (1ull << 32) | hk, __ATOMIC_RELAXED,
-                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
-                return;
-            w = expect;  // claimed meanwhile: look at it again
-            continue;
-        }
+        uint64_t w = 0;  // empty: claim it for this key with count 1
+        if (__hip_atomic_compare_exchange_strong(table + h, &w, (1ull << 32) | hk, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP))
+            return;
         if ((uint32_t)w == hk) {
             __hip_atomic_fetch_add(table + h, 1ull << 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             return;
         }
         h = (h + 1) & (uint32_t)(kJoinSlots - 1);
-        w = __hip_atomic_load(table + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 }
 
