@@ -46,17 +46,20 @@ __device__ __forceinline__ uint32_t join_slot(uint32_t hk, int pbits) {
 
 // A claim is tried first: on an empty slot (most of them, at a load factor
 // of at most 1/2) it is the only LDS operation; a failed claim returns the
-// slot's word, whose key never changes once set.
-__device__ __forceinline__ void join_insert(uint64_t* table, uint32_t hk, int pbits) {
+// slot's word, whose key never changes once set.  Returns the slot this
+// call claimed, or kNoClaim when the key was counted into a slot that
+// another call had claimed.
+constexpr uint32_t kNoClaim = 0xFFFFFFFFu;
+__device__ __forceinline__ uint32_t join_insert(uint64_t* table, uint32_t hk, int pbits) {
     uint32_t h = join_slot(hk, pbits);
     while (true) {
         uint64_t w = 0;  // empty: claim it for this key with count 1
         if (__hip_atomic_compare_exchange_strong(table + h, &w, (1ull << 32) | hk, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_WORKGROUP))
-            return;
+            return h;
         if ((uint32_t)w == hk) {
             __hip_atomic_fetch_add(table + h, 1ull << 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            return;
+            return kNoClaim;
         }
         h = (h + 1) & (uint32_t)(kJoinSlots - 1);
     }
@@ -132,6 +135,10 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe(const uint32_t* __res
     JoinPair cur = join_pair(b, rk, rn, rs, sk, sn, ss, B);
     uint32_t bk[kJoinPre], pk[kJoinPre];
     if (cur.held()) join_load(cur, bk, pk);
+    // after a held pair the only nonzero slots are the ones its lanes
+    // claimed (recorded in claimed[]): the next held pair clears just those
+    bool sparse = false;
+    uint32_t claimed[kJoinPre];
     for (; b < B; b += gridDim.x) {
         const int nb = b + (int)gridDim.x;
         JoinPair nxt = cur;
@@ -139,12 +146,20 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe(const uint32_t* __res
         uint32_t nbk[kJoinPre], npk[kJoinPre];
         __syncthreads();  // the previous pair's lookups are done before the table is cleared
         if (cur.held()) {
-            for (int i = threadIdx.x; i < kJoinSlots; i += kJoinBlock) table[i] = 0ull;
+            if (sparse) {
+                #pragma unroll
+                for (int q = 0; q < kJoinPre; ++q)
+                    if (claimed[q] != kNoClaim) table[claimed[q]] = 0ull;
+            } else {
+                for (int i = threadIdx.x; i < kJoinSlots; i += kJoinBlock) table[i] = 0ull;
+            }
             if (nb < B && nxt.held()) join_load(nxt, nbk, npk);
             __syncthreads();
             #pragma unroll
             for (int q = 0; q < kJoinPre; ++q)
-                if (cur.b0 + threadIdx.x + (int64_t)q * kJoinBlock < cur.b1) join_insert(table, bk[q], pbits);
+                claimed[q] = cur.b0 + threadIdx.x + (int64_t)q * kJoinBlock < cur.b1 ? join_insert(table, bk[q], pbits)
+                                                                                     : kNoClaim;
+            sparse = true;
             __syncthreads();
             #pragma unroll
             for (int q = 0; q < kJoinPre; ++q)
@@ -153,6 +168,7 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_probe(const uint32_t* __res
         } else {
             if (nb < B && nxt.held()) join_load(nxt, nbk, npk);
             if (cur.b1 > cur.b0 && cur.p1 > cur.p0) {
+                sparse = false;  // the chunk fills leave slots no lane recorded
                 for (int64_t c0 = cur.b0; c0 < cur.b1; c0 += kJoinChunk) {
                     const int64_t c1 = std::min<int64_t>(c0 + kJoinChunk, cur.b1);
                     for (int i = threadIdx.x; i < kJoinSlots; i += kJoinBlock) table[i] = 0ull;

@@ -633,6 +633,24 @@ def test_hash_join_skew_and_special_keys(torch_gpu, ctx, orc):
     assert int(cnt.item()) == orc.join_count(r, s)
 
 
+def test_hash_join_heavy_partitions_among_light_ones(torch_gpu, ctx, orc):
+    # 2048 partitions, 8 per workgroup: the 64 partitions of the heavy keys
+    # (9 000 copies on each side) need chunked table fills, the others are
+    # filled from registers, so workgroups switch between the two kinds
+    torch = torch_gpu
+    rng = np.random.default_rng(2048)
+    heavy = rng.choice(1 << 30, 64, replace=False).astype(np.int64) + (1 << 30)
+    r = np.concatenate([rng.permutation(1 << 23)[: 1 << 22], np.repeat(heavy, 9000)])
+    s = np.concatenate([rng.permutation(1 << 23)[: 1 << 22], np.repeat(heavy, 9000)])
+    r = rng.permutation(r).astype(np.int32)
+    s = rng.permutation(s).astype(np.int32)
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    ctx.join_count(dev(torch, r), dev(torch, s), cnt)
+    want = orc.join_count(r, s)
+    assert want > 64 * 9000 * 9000
+    assert int(cnt.item()) == want
+
+
 @pytest.mark.parametrize("n", [(1 << 25) + 3, 1 << 26])
 def test_hash_join_full_size_property(torch_gpu, ctx, n):
     # R = a random permutation of [0, n), S = R's first n//2 keys shifted by n
